@@ -1,0 +1,128 @@
+/*
+ * tnsgpu.h -- C ABI of libtnsgpu.so, the MI355X (gfx950) block-sparse tensor
+ * engine.  Plain pointers and sizes only; device buffers belong to the caller
+ * (the Python package allocates them with torch), the library never hands out
+ * device memory.  Every function returns 0 on success; on failure it returns
+ * a non-zero code and tns_last_error() gives a thread-local message.
+ *
+ * Reference interfaces replaced (paths relative to the TNSP repository root):
+ *   shapes       TAT::Tensor(names, edges)          PyTAT/PyTAT.hpp:774-779,
+ *                                                   TAT/include/TAT/structure/core.hpp:152-191
+ *   edge op      Tensor::edge_operator               PyTAT/PyTAT.hpp:1000-1015,
+ *                                                   TAT/include/TAT/implement/edge_operator.hpp:35-692
+ *   contract     Tensor::contract                    PyTAT/PyTAT.hpp:988-1004,
+ *                                                   TAT/include/TAT/implement/contract.hpp:307-620
+ *   svd          Tensor::svd                         PyTAT/PyTAT.hpp:1031-1058,
+ *                                                   TAT/include/TAT/implement/svd.hpp:260-538
+ *   qr           Tensor::qr                          PyTAT/PyTAT.hpp:1059-1074,
+ *                                                   TAT/include/TAT/implement/qr.hpp:310-508
+ *   conjugate    Tensor::conjugate                   PyTAT/PyTAT.hpp:1029,
+ *                                                   TAT/include/TAT/implement/conjugate.hpp:32-120
+ *   amplitude    Configuration.hole(()) / replace    tetragono/tetragono/auxiliaries/single_layer_auxiliaries.py:463-498,
+ *                                                   500-720
+ *
+ * Names are int32 ids (the caller interns strings).  Symmetry kinds:
+ *   0 No, 1 BoseZ2, 2 BoseU1, 3 FermiU1, 4 FermiZ2, 5 FermiU1BoseZ2,
+ *   6 FermiU1BoseU1, 7 FermiU1FermiU1.
+ * A symmetry value is 2 int32 (unused components 0; Z2 as 0/1).
+ * dtype: 0 = float64, 1 = float32.
+ */
+#ifndef TNSGPU_H
+#define TNSGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tns_shape tns_shape;
+typedef struct tns_plan tns_plan;
+
+/* library */
+const char* tns_last_error(void);
+const char* tns_version(void);
+int tns_device_count(int* count);
+
+/* shapes (TAT::Tensor(names, edges): block layout of core.hpp:152-191) */
+int tns_shape_create(int kind, int rank, const int32_t* names, const int32_t* nseg, const int32_t* syms,
+                     const int64_t* dims, const uint8_t* arrows, tns_shape** out);
+int tns_shape_free(tns_shape* s);
+int tns_shape_counts(const tns_shape* s, int32_t* kind, int32_t* rank, int32_t* total_segments, int64_t* nblocks,
+                     int64_t* size);
+int tns_shape_export(const tns_shape* s, int32_t* names, int32_t* nseg, int32_t* syms, int64_t* dims, uint8_t* arrows);
+/* block offsets in storage order: positions (nblocks x rank) and offsets (nblocks) */
+int tns_shape_blocks(const tns_shape* s, int64_t* positions, int64_t* offsets);
+int tns_shape_rename(const tns_shape* s, int n, const int32_t* old_names, const int32_t* new_names, tns_shape** out);
+
+/* plans.  Output shapes are returned as new handles owned by the caller. */
+int tns_plan_edge_operator(const tns_shape* src, int n_split, const int32_t* split_old, const int32_t* split_count,
+                           const int32_t* split_names, const tns_shape* split_edges, int n_rev, const int32_t* rev,
+                           int n_merge, const int32_t* merge_new, const int32_t* merge_count, const int32_t* merge_old,
+                           int n_new, const int32_t* new_names, int apply_parity, int n_ex_split,
+                           const int32_t* ex_split, int n_ex_rb, const int32_t* ex_rb, int n_ex_ra,
+                           const int32_t* ex_ra, int n_ex_merge, const int32_t* ex_merge, tns_plan** out);
+int tns_plan_contract(const tns_shape* a, const tns_shape* b, int npairs, const int32_t* pairs, tns_plan** out);
+int tns_plan_svd(const tns_shape* t, int nfree, const int32_t* free_u, int32_t cu, int32_t cv, int32_t su,
+                 int32_t sv, tns_plan** out);
+int tns_plan_qr(const tns_shape* t, char direction, int nfree, const int32_t* free_names, int32_t cq, int32_t cr,
+                tns_plan** out);
+int tns_plan_conjugate(const tns_shape* t, int trivial_metric, tns_plan** out);
+int tns_plan_free(tns_plan* p);
+
+/* number of outputs and each output shape (for svd: after tns_svd_factor) */
+int tns_plan_num_outputs(const tns_plan* p, int* n);
+int tns_plan_output_shape(const tns_plan* p, int which, tns_shape** out);
+/* device workspace the caller must provide to run the plan */
+int tns_plan_workspace(const tns_plan* p, int dtype, int64_t* bytes);
+/* flop count of the plan (sector GEMMs, QR/SVD counted as in LAPACK) */
+int tns_plan_flops(const tns_plan* p, int64_t* flops);
+
+/* execute on a HIP stream (NULL = default stream) */
+int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const* outputs, void* workspace,
+                 void* stream);
+/* svd phase 1: merge + per-sector SVD, then (synchronising) the TAT truncation
+ * rule (svd.hpp:430-481): remain_cut < 0 means no count limit, relative_cut
+ * <= 0 means none.  Output shapes are known afterwards; tns_plan_run then
+ * writes U, S, V. */
+int tns_svd_factor(tns_plan* p, int dtype, const void* input, void* workspace, int64_t remain_cut,
+                   double relative_cut, void* stream);
+
+/* batched amplitude engine: psi(s) of many configurations of one PEPS in
+ * lockstep (tetragono's single-layer boundary MPS,
+ * tetragono/tetragono/auxiliaries/single_layer_auxiliaries.py:463-498,
+ * 710-720).  sites: L1*L2 raw site shapes (with the physical edge) and their
+ * device data; name_ids: P0 T U D L R L1 L2 L3 R1 R2 P Q then the L1*L2
+ * projected physical names P_{l1}_{l2}_0 row-major.  occupations: nconf x
+ * L1 x L2 indices along the physical edge.  psi: host output (nconf).
+ * The engine owns its device arenas until tns_engine_free. */
+typedef struct tns_engine tns_engine;
+int tns_engine_create(int kind, int L1, int L2, const tns_shape* const* sites, const void* const* site_data,
+                      int dtype, const int32_t* name_ids, int n_names, int threads, tns_engine** out);
+int tns_engine_set_chunk(tns_engine* e, int64_t chunk);
+int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
+                        double relative_cut, double* psi, void* stream);
+/* values: plan_s exec_s total_s gemm_flops qr_flops svd_flops perm_elems
+ * launches device_bytes configs (n <= 10) */
+int tns_engine_stats(const tns_engine* e, double* values, int n);
+int tns_engine_reset_stats(tns_engine* e);
+int tns_engine_free(tns_engine* e);
+
+/* planner introspection used by the CPU test-suite: element counts of the
+ * plan's logical buffers (-1 = caller-owned input), the sector table of an
+ * svd plan, and the truncation rule applied to host-computed singular
+ * values (the GPU path applies the same rule inside tns_svd_factor) */
+int tns_plan_buffers(const tns_plan* p, int64_t* sizes, int* n);
+int tns_svd_sectors(const tns_plan* p, int64_t* k, int64_t* off, int32_t* syms, int* n);
+int tns_svd_truncate(tns_plan* p, const double* svals, int64_t remain_cut, double relative_cut);
+
+/* planner introspection used by the CPU test-suite: task records of a stage
+ * (stage ids and record layouts are documented in capi.cpp) */
+int tns_plan_stage_count(const tns_plan* p, int stage, int64_t* n);
+int tns_plan_stage_export(const tns_plan* p, int stage, int64_t* records);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNSGPU_H */

@@ -1,0 +1,39 @@
+"""The C ABI library loads (no GPU needed) and exports every declared symbol."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "tnsgpu.h")).read()
+    return sorted(set(re.findall(r"\b(tns_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from tnsp_amd import _lib
+
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the ctypes layer binds exactly the declared surface
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_error_path_reports_message():
+    from tnsp_amd import _lib
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    names = _lib.arr32([0, 0])
+    nseg = _lib.arr32([1, 1])
+    syms = _lib.arr32([0, 0, 0, 0])
+    dims = _lib.arr64([2, 2])
+    rc = L.tns_shape_create(3, 2, names, nseg, syms, dims, None, ctypes.byref(h))
+    assert rc != 0
+    assert b"duplicated" in L.tns_last_error()
+    rc = L.tns_shape_create(42, 0, names, nseg, syms, dims, None, ctypes.byref(h))
+    assert rc != 0 and b"kind" in L.tns_last_error()

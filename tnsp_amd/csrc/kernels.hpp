@@ -1,0 +1,73 @@
+// Device task layouts and kernel launchers (implemented in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tns {
+
+constexpr int DEV_MAXR = 8;
+
+// One strided block copy: dst[i] = (neg ? -1 : 1) * src[i] over a rank-r box,
+// i enumerated row-major over dims (destination order).
+struct PermDev {
+    const void* src;
+    void* dst;
+    int64_t n;
+    int32_t rank, neg;
+    int32_t dims[DEV_MAXR];
+    int32_t ss[DEV_MAXR];
+    int32_t ds[DEV_MAXR];
+};
+
+struct GemmDev {
+    const void* a;
+    const void* b;
+    void* c;
+    int32_t m, n, k, lda, ldb, ldc, ta, tb;
+    double alpha;
+};
+
+struct TileDev {
+    int32_t task, tm, tn, pad;
+};
+
+struct FillDev {
+    void* p;
+    int64_t n;
+};
+
+struct QrDev {
+    const void* in;
+    void* f1;
+    void* f2;
+    double* work;  // scratch: mx*nx + mx*k + k doubles
+    int32_t m, n, lq, pad;
+};
+
+struct SvdDev {
+    const void* in;
+    void* u;
+    double* s;  // singular values are always stored in double
+    void* v;
+    double* work;  // scratch: mx*kx + kx*kx + 2*kx doubles
+    int32_t m, n, pad0, pad1;
+};
+
+// dtype: 0 = float64, 1 = float32
+// Element chunk per workgroup of the permute kernel.
+constexpr int PERM_CHUNK = 2048;
+void launch_permute(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg, int dtype,
+                    hipStream_t stream);
+// same copy with a float64 source (singular values into S)
+void launch_permute_from_f64(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg,
+                             int dtype, hipStream_t stream);
+void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg, int dtype,
+                 hipStream_t stream);
+constexpr int GEMM_TILE = 32;
+void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream);
+void launch_qr(const QrDev* tasks, int64_t num, int dtype, hipStream_t stream);
+void launch_svd(const SvdDev* tasks, int64_t num, int dtype, hipStream_t stream);
+
+}  // namespace tns

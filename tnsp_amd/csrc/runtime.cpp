@@ -1,0 +1,290 @@
+// Launch batching and staging of task arrays (see runtime.hpp).
+#include "runtime.hpp"
+
+#include <cstring>
+#include <stdexcept>
+
+namespace tns {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Staging& Staging::get() {
+    static Staging s;
+    return s;
+}
+
+char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
+    const size_t n = blob.size();
+    if (n == 0) return dev_;
+    if (!ev_) check_hip(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "event create");
+    // the pinned staging buffer may still feed the previous copy
+    if (pending_) {
+        check_hip(hipEventSynchronize(ev_), "staging wait");
+        pending_ = false;
+    }
+    if (n > host_cap_) {
+        if (host_) check_hip(hipHostFree(host_), "host free");
+        host_cap_ = std::max(n, host_cap_ * 2);
+        check_hip(hipHostMalloc(reinterpret_cast<void**>(&host_), host_cap_, hipHostMallocDefault), "host alloc");
+    }
+    if (n > dev_cap_) {
+        // kernels queued earlier may still read the old task buffer
+        check_hip(hipDeviceSynchronize(), "device synchronize");
+        if (dev_) check_hip(hipFree(dev_), "device free");
+        dev_cap_ = std::max(n, dev_cap_ * 2);
+        check_hip(hipMalloc(reinterpret_cast<void**>(&dev_), dev_cap_), "device alloc");
+    }
+    std::memcpy(host_, blob.data(), n);
+    check_hip(hipMemcpyAsync(dev_, host_, n, hipMemcpyHostToDevice, st), "task upload");
+    check_hip(hipEventRecord(ev_, st), "event record");
+    pending_ = true;
+    return dev_;
+}
+
+template<typename T>
+size_t Batch::put(const std::vector<T>& v) {
+    const size_t off = (blob_.size() + 255) / 256 * 256;
+    blob_.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(blob_.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+namespace {
+PermDev bind_perm(const PermTask& t, void* const* bufs, int64_t es_src, int64_t es_dst) {
+    PermDev d{};
+    d.src = static_cast<const char*>(bufs[t.src_buf]) + t.src * es_src;
+    d.dst = static_cast<char*>(bufs[t.dst_buf]) + t.dst * es_dst;
+    d.n = t.n;
+    d.rank = t.rank;
+    d.neg = t.neg;
+    for (int i = 0; i < MAXR; i++) {
+        d.dims[i] = t.dims[i];
+        d.ss[i] = t.ss[i];
+        d.ds[i] = t.ds[i];
+    }
+    return d;
+}
+}  // namespace
+
+void Batch::perm(const std::vector<PermTask>& ts, void* const* bufs, int64_t es) {
+    for (auto& t : ts) perm_one(t, bufs, es);
+}
+
+void Batch::perm_one(const PermTask& t, void* const* bufs, int64_t es) {
+    if (t.n <= 0) return;
+    pperm_.push_back(bind_perm(t, bufs, es, es));
+    perm_elems_ += t.n;
+}
+
+void Batch::diag(const std::vector<PermTask>& ts, void* const* bufs, int64_t es) {
+    for (auto& t : ts)
+        if (t.n > 0) pdiag_.push_back(bind_perm(t, bufs, 8, es));
+}
+
+void Batch::flush_diag() {
+    std::vector<FillDev> none;
+    flush_flat(DIAG, pdiag_, none);
+}
+
+void Batch::fill(const std::vector<FillTask>& ts, void* const* bufs, int64_t es) {
+    for (auto& t : ts)
+        if (t.n > 0) pfill_.push_back(FillDev{static_cast<char*>(bufs[t.buf]) + t.off * es, t.n});
+}
+
+void Batch::fill_raw(void* p, int64_t n) {
+    if (n > 0) pfill_.push_back(FillDev{p, n});
+}
+
+void Batch::flush_flat(Kind kind, std::vector<PermDev>& pd, std::vector<FillDev>& fd) {
+    const bool isfill = kind == FILL;
+    const size_t T = isfill ? fd.size() : pd.size();
+    if (T == 0) return;
+    std::vector<int64_t> prefix(T + 1, 0);
+    for (size_t i = 0; i < T; i++) prefix[i + 1] = prefix[i] + (isfill ? fd[i].n : pd[i].n);
+    const int64_t total = prefix[T];
+    const int64_t nwg = (total + PERM_CHUNK - 1) / PERM_CHUNK;
+    if (nwg >= (int64_t(1) << 31)) throw Error("permute launch too large");
+    std::vector<int32_t> wg(nwg + 1);
+    size_t t = 0;
+    for (int64_t w = 0; w < nwg; w++) {
+        const int64_t e = w * PERM_CHUNK;
+        while (t + 1 < T && prefix[t + 1] <= e) t++;
+        wg[w] = int32_t(t);
+    }
+    wg[nwg] = int32_t(T);
+    Cmd c;
+    c.kind = kind;
+    c.tasks = isfill ? put(fd) : put(pd);
+    c.prefix = put(prefix);
+    c.wg = put(wg);
+    c.count = int64_t(T);
+    c.nwg = nwg;
+    cmds_.push_back(c);
+    pd.clear();
+    fd.clear();
+}
+
+void Batch::flush_perm() {
+    std::vector<FillDev> none;
+    flush_flat(PERM, pperm_, none);
+}
+
+void Batch::flush_fill() {
+    std::vector<PermDev> none;
+    flush_flat(FILL, none, pfill_);
+}
+
+void Batch::gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es) {
+    for (auto& t : ts) {
+        if (t.m <= 0 || t.n <= 0 || t.k <= 0) continue;
+        GemmDev g{};
+        g.a = static_cast<const char*>(bufs[t.a_buf]) + t.a * es;
+        g.b = static_cast<const char*>(bufs[t.b_buf]) + t.b * es;
+        g.c = static_cast<char*>(bufs[t.c_buf]) + t.c * es;
+        g.m = t.m;
+        g.n = t.n;
+        g.k = t.k;
+        g.lda = t.lda;
+        g.ldb = t.ldb;
+        g.ldc = t.ldc;
+        g.ta = t.ta;
+        g.tb = t.tb;
+        g.alpha = t.alpha;
+        const int32_t id = int32_t(pgemm_.size());
+        pgemm_.push_back(g);
+        const int tm = (t.m + GEMM_TILE - 1) / GEMM_TILE, tn = (t.n + GEMM_TILE - 1) / GEMM_TILE;
+        for (int i = 0; i < tm; i++)
+            for (int j = 0; j < tn; j++) ptile_.push_back(TileDev{id, i, j, 0});
+    }
+}
+
+void Batch::flush_gemm() {
+    if (ptile_.empty()) {
+        pgemm_.clear();
+        return;
+    }
+    Cmd c;
+    c.kind = GEMM;
+    c.tasks = put(pgemm_);
+    c.prefix = put(ptile_);
+    c.wg = 0;
+    c.count = int64_t(pgemm_.size());
+    c.nwg = int64_t(ptile_.size());
+    cmds_.push_back(c);
+    pgemm_.clear();
+    ptile_.clear();
+}
+
+void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, double* scratch) {
+    double* cur = scratch;
+    for (auto& t : ts) {
+        QrDev q{};
+        q.in = static_cast<const char*>(bufs[t.in_buf]) + t.in * es;
+        q.f1 = static_cast<char*>(bufs[t.f1_buf]) + t.f1 * es;
+        q.f2 = static_cast<char*>(bufs[t.f2_buf]) + t.f2 * es;
+        q.work = cur;
+        q.m = t.m;
+        q.n = t.n;
+        q.lq = t.lq;
+        const int64_t mx = t.lq ? t.n : t.m, nx = t.lq ? t.m : t.n, k = std::min(mx, nx);
+        cur += mx * nx + mx * k + k;
+        pqr_.push_back(q);
+    }
+}
+
+void Batch::flush_qr() {
+    if (pqr_.empty()) return;
+    Cmd c;
+    c.kind = QR;
+    c.tasks = put(pqr_);
+    c.prefix = c.wg = 0;
+    c.count = c.nwg = int64_t(pqr_.size());
+    cmds_.push_back(c);
+    pqr_.clear();
+}
+
+void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch) {
+    double* cur = scratch;
+    for (auto& t : ts) {
+        SvdDev s{};
+        s.in = static_cast<const char*>(bufs[t.in_buf]) + t.in * es;
+        s.u = static_cast<char*>(bufs[t.u_buf]) + t.u * es;
+        s.s = reinterpret_cast<double*>(static_cast<char*>(bufs[t.s_buf]) + t.s * 8);
+        s.v = static_cast<char*>(bufs[t.v_buf]) + t.v * es;
+        s.work = cur;
+        s.m = t.m;
+        s.n = t.n;
+        const int64_t mx = std::max(t.m, t.n), kx = std::min(t.m, t.n);
+        cur += mx * kx + kx * kx + kx;
+        psvd_.push_back(s);
+    }
+}
+
+void Batch::flush_svd() {
+    if (psvd_.empty()) return;
+    Cmd c;
+    c.kind = SVD;
+    c.tasks = put(psvd_);
+    c.prefix = c.wg = 0;
+    c.count = c.nwg = int64_t(psvd_.size());
+    cmds_.push_back(c);
+    psvd_.clear();
+}
+
+void Batch::run(hipStream_t st) {
+    flush_diag();
+    flush_perm();
+    flush_fill();
+    flush_gemm();
+    flush_qr();
+    flush_svd();
+    if (cmds_.empty()) return;
+    char* base = Staging::get().upload(blob_, st);
+    for (auto& c : cmds_) {
+        switch (c.kind) {
+            case PERM:
+                launch_permute(reinterpret_cast<const PermDev*>(base + c.tasks),
+                               reinterpret_cast<const int64_t*>(base + c.prefix),
+                               reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
+                break;
+            case DIAG:
+                launch_permute_from_f64(reinterpret_cast<const PermDev*>(base + c.tasks),
+                                        reinterpret_cast<const int64_t*>(base + c.prefix),
+                                        reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
+                break;
+            case FILL:
+                launch_fill(reinterpret_cast<const FillDev*>(base + c.tasks),
+                            reinterpret_cast<const int64_t*>(base + c.prefix),
+                            reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
+                break;
+            case GEMM:
+                launch_gemm(reinterpret_cast<const GemmDev*>(base + c.tasks),
+                            reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, dtype_, st);
+                break;
+            case QR:
+                launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, dtype_, st);
+                break;
+            case SVD:
+                launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, dtype_, st);
+                break;
+        }
+    }
+    reset();
+}
+
+void Batch::reset() {
+    blob_.clear();
+    cmds_.clear();
+    pperm_.clear();
+    pdiag_.clear();
+    pfill_.clear();
+    pgemm_.clear();
+    ptile_.clear();
+    pqr_.clear();
+    psvd_.clear();
+    perm_elems_ = 0;
+}
+
+}  // namespace tns

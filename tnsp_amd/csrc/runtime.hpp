@@ -1,0 +1,86 @@
+// Launch batching: task lists from any number of plans (and, in the batched
+// amplitude engine, from any number of configurations) are bound to device
+// pointers, packed into one host blob, uploaded with one copy and executed as
+// a short sequence of kernel launches on one stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+#include "kernels.hpp"
+
+namespace tns {
+
+void check_hip(hipError_t e, const char* what);
+
+// Device/pinned staging for task arrays.  Not thread-safe; one user at a time.
+class Staging {
+   public:
+    static Staging& get();
+    // copies blob to device memory on stream st and returns the device base
+    char* upload(const std::vector<char>& blob, hipStream_t st);
+
+   private:
+    char* dev_ = nullptr;
+    size_t dev_cap_ = 0;
+    char* host_ = nullptr;
+    size_t host_cap_ = 0;
+    hipEvent_t ev_ = nullptr;
+    bool pending_ = false;
+};
+
+class Batch {
+   public:
+    explicit Batch(int dtype) : dtype_(dtype) {}
+    void perm(const std::vector<PermTask>& ts, void* const* bufs, int64_t es);
+    void perm_one(const PermTask& t, void* const* bufs, int64_t es);
+    void flush_perm();
+    void fill(const std::vector<FillTask>& ts, void* const* bufs, int64_t es);
+    void fill_raw(void* p, int64_t n);
+    void flush_fill();
+    void gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es);
+    void flush_gemm();
+    void qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, double* scratch);
+    void flush_qr();
+    void svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch);
+    void flush_svd();
+    // copies of singular values (double, buffer 4) into the diagonal of S
+    void diag(const std::vector<PermTask>& ts, void* const* bufs, int64_t es);
+    void flush_diag();
+    void svals_to_diag(const std::vector<PermTask>& ts, void* const* bufs, int64_t es) {
+        diag(ts, bufs, es);
+        flush_diag();
+    }
+    void run(hipStream_t st);
+    bool empty() const { return cmds_.empty(); }
+    int64_t perm_elements() const { return perm_elems_; }
+    void reset();
+
+   private:
+    enum Kind { PERM, FILL, GEMM, QR, SVD, DIAG };
+    struct Cmd {
+        Kind kind;
+        size_t tasks, prefix, wg;  // blob offsets
+        int64_t count, nwg;
+    };
+    template<typename T>
+    size_t put(const std::vector<T>& v);
+    void flush_flat(Kind kind, std::vector<PermDev>& pd, std::vector<FillDev>& fd);
+
+    int dtype_;
+    std::vector<char> blob_;
+    std::vector<Cmd> cmds_;
+    std::vector<PermDev> pperm_, pdiag_;
+    std::vector<FillDev> pfill_;
+    std::vector<GemmDev> pgemm_;
+    std::vector<TileDev> ptile_;
+    std::vector<QrDev> pqr_;
+    std::vector<SvdDev> psvd_;
+    int64_t perm_elems_ = 0;
+    double* scratch_cursor_ = nullptr;
+};
+
+}  // namespace tns

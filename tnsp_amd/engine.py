@@ -1,0 +1,68 @@
+"""Python handle of the batched amplitude engine (libtnsgpu ``tns_engine_*``).
+
+One engine holds one PEPS.  ``evaluate(configs)`` returns psi(s) for every
+configuration: the native engine steps all of them through tetragono's
+single-layer boundary-MPS op sequence in lockstep
+(tetragono/tetragono/auxiliaries/single_layer_auxiliaries.py:463-498 per row,
+:254-312 and :710-720 for the final row), planning each op per configuration
+on host threads and executing one batched launch per kernel kind.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from .plan import DTYPE_CODE, cut_params
+from .shape import NAMES
+from .tensor import _stream
+
+STAT_NAMES = ["plan_s", "exec_s", "total_s", "gemm_flops", "qr_flops", "svd_flops", "perm_elems", "launches",
+              "device_bytes", "configs"]
+
+
+class AmplitudeEngine:
+    def __init__(self, lattice, threads=None, chunk=1024):
+        spec = lattice.spec
+        self.lattice = lattice  # keeps the site buffers alive
+        self.spec = spec
+        sites = [lattice.sites[l1][l2] for l1 in range(spec.L1) for l2 in range(spec.L2)]
+        shapes = (ctypes.c_void_p * len(sites))(*[t._shape.handle for t in sites])
+        datas = (ctypes.c_void_p * len(sites))(*[t._data.data_ptr() for t in sites])
+        names = ["P0", "T", "U", "D", "L", "R", "L1", "L2", "L3", "R1", "R2", "P", "Q"]
+        names += [f"P_{l1}_{l2}_0" for l1 in range(spec.L1) for l2 in range(spec.L2)]
+        if threads is None:
+            threads = int(os.environ.get("TNS_THREADS", min(16, os.cpu_count() or 1)))
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tns_engine_create(sites[0].model.kind.id, spec.L1, spec.L2, shapes, datas,
+                                                DTYPE_CODE[lattice.dtype], _lib.arr32(NAMES.ids(names)), len(names),
+                                                int(threads), ctypes.byref(h)))
+        self.handle = h
+        self.set_chunk(chunk)
+
+    def set_chunk(self, chunk):
+        _lib.check(_lib.lib().tns_engine_set_chunk(self.handle, int(chunk)))
+
+    def evaluate(self, configs, cut=None):
+        """psi for configurations of shape (n, L1, L2) (physical indices)."""
+        cfg = np.ascontiguousarray(np.asarray(configs), dtype=np.int32)
+        n = cfg.shape[0]
+        remain, relative = cut_params(self.spec.cut if cut is None else cut)
+        psi = np.zeros(n, dtype=np.float64)
+        _lib.check(_lib.lib().tns_engine_evaluate(self.handle, n, cfg.ctypes.data_as(_lib.c_i32p), remain, relative,
+                                                  psi.ctypes.data_as(_lib.c_dblp), ctypes.c_void_p(_stream())))
+        return psi
+
+    def stats(self):
+        vals = (ctypes.c_double * len(STAT_NAMES))()
+        _lib.check(_lib.lib().tns_engine_stats(self.handle, vals, len(STAT_NAMES)))
+        return dict(zip(STAT_NAMES, list(vals)))
+
+    def reset_stats(self):
+        _lib.check(_lib.lib().tns_engine_reset_stats(self.handle))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.tns_engine_free(h)
+            self.handle = None
