@@ -106,6 +106,11 @@ int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations
  * launches device_bytes configs (n <= 10) */
 int tns_engine_stats(const tns_engine* e, double* values, int n);
 int tns_engine_reset_stats(tns_engine* e);
+/* per-kernel-kind device timing with HIP events on the launch stream:
+ * kinds permute, fill, gemm, qr, svd, diag; per kind: launches, total ms,
+ * algorithmic work (bytes for permute/fill/diag, flops for gemm/qr/svd) */
+int tns_engine_set_profile(tns_engine* e, int on);
+int tns_engine_kernel_stats(const tns_engine* e, double* values, int n);
 int tns_engine_free(tns_engine* e);
 
 /* planner introspection used by the CPU test-suite: element counts of the

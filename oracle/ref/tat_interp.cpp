@@ -28,8 +28,12 @@
 //   amplitude <out> <L1> <L2> <cut> <prefix>   sites <prefix>_<l1>_<l2> already
 //       shrunk (physical legs contracted away); computes psi with tetragono's
 //       contraction order and stores it as tensor <out>.
-//   amplitude_bench <L1> <L2> <cut> <nconf> <repeat>  then per site and config
-//       the raw site tensors <prefix>...: see bench notes in oracle/ref/README
+//   time_amplitude <L1> <L2> <cut> <nconf> <prefix>   shrunk sites
+//       <prefix>_<c>_<l1>_<l2>; prints "time <seconds> <sum |psi|>"
+//   time_raw <L1> <L2> <cut> <prefix> <nconf> occ...   raw sites
+//       <prefix>_<l1>_<l2> (with P0); projects each configuration with a
+//       one-hot shrinker (sampling_lattice/lattice.py:291-339) and evaluates
+//       psi; projection and amplitude are both inside the timed region
 #include <chrono>
 #include <cstdio>
 #include <iostream>
@@ -147,6 +151,17 @@ struct Interp {
             dl[i - 1] = safe_contract(safe_contract(dl[i - 1], u, {{right, left}}), s, {{right, left}});
         }
         return dl;
+    }
+
+    // sampling_lattice/lattice.py:291-339: contract P0 with a one-hot shrinker
+    static T shrink(const T& site, int l1, int l2, long long occupation) {
+        const auto& phys = site.edges("P0");
+        auto q = make_sym(occupation);
+        T sh({"P", "Q"}, {E({{q, 1}}, false), phys.conjugate()});
+        sh.zero_();
+        sh.at(std::unordered_map<Name, std::pair<Sym, TAT::Size>>{{"Q", {-q, 0}}, {"P", {q, 0}}}) = 1;
+        auto renamed = sh.edge_rename({{"P", "P_" + std::to_string(l1) + "_" + std::to_string(l2) + "_0"}});
+        return site.contract(renamed, {{"P0", "Q"}});
     }
 
     // hole(()) with hint ("H", 0): single_layer_auxiliaries.py:710-720 ->
@@ -339,6 +354,29 @@ struct Interp {
                 double acc = 0;
                 for (int c = 0; c < nconf; c++) {
                     auto psi = amplitude(lats[c], L1, L2, cut);
+                    acc += std::abs(psi.storage().size() ? psi.storage()[0] : 0.0);
+                }
+                auto t1 = std::chrono::steady_clock::now();
+                std::printf("time %.9f %.17g\n", std::chrono::duration<double>(t1 - t0).count(), acc);
+            } else if (cmd == "time_raw") {
+                // time_raw <L1> <L2> <cut> <prefix> <nconf> then nconf*L1*L2
+                // occupations: raw sites <prefix>_<l1>_<l2>; projection and
+                // amplitude both timed; prints seconds and sum |psi|
+                std::string prefix;
+                int L1, L2, nconf;
+                double cut;
+                in >> L1 >> L2 >> cut >> prefix >> nconf;
+                std::vector<int> occ(size_t(nconf) * L1 * L2);
+                for (auto& o : occ) in >> o;
+                auto t0 = std::chrono::steady_clock::now();
+                double acc = 0;
+                for (int c = 0; c < nconf; c++) {
+                    std::vector<std::vector<T>> lat(L1);
+                    for (int l1 = 0; l1 < L1; l1++)
+                        for (int l2 = 0; l2 < L2; l2++)
+                            lat[l1].push_back(shrink(env.at(prefix + "_" + std::to_string(l1) + "_" + std::to_string(l2)),
+                                                     l1, l2, occ[(size_t(c) * L1 + l1) * L2 + l2]));
+                    auto psi = amplitude(lat, L1, L2, cut);
                     acc += std::abs(psi.storage().size() ? psi.storage()[0] : 0.0);
                 }
                 auto t1 = std::chrono::steady_clock::now();

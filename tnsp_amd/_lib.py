@@ -55,6 +55,8 @@ _SIGNATURES = {
     "tns_engine_evaluate": [c_vp, ctypes.c_int64, c_i32p, ctypes.c_int64, ctypes.c_double, c_dblp, c_vp],
     "tns_engine_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_reset_stats": [c_vp],
+    "tns_engine_set_profile": [c_vp, ctypes.c_int],
+    "tns_engine_kernel_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_free": [c_vp],
     "tns_plan_stage_count": [c_vp, ctypes.c_int, c_i64p],
     "tns_plan_stage_export": [c_vp, ctypes.c_int, c_i64p],

@@ -134,6 +134,7 @@ void Engine::grow_all(hipStream_t st) {
 void Engine::run_batch(Batch& b, hipStream_t st) {
     const double t0 = now();
     stats_.perm_elems += double(b.perm_elements());
+    if (profile_) b.set_timer(&timer_);
     b.run(st);
     stats_.exec_s += now() - t0;
 }
@@ -333,6 +334,7 @@ void Engine::svd(std::vector<Slot*>& t, const std::vector<int32_t>& free_u, int3
     if (sv_bytes) check_hip(hipMemcpyAsync(svals_host_.data(), S, sv_bytes, hipMemcpyDeviceToHost, st), "svals copy");
     check_hip(hipStreamSynchronize(st), "svals sync");
     stats_.exec_s += now() - t1s;
+    if (profile_) timer_.collect();
     t0 = now();
     pool_.parallel_for(B, [&](int64_t b, int64_t e) {
         for (int64_t i = b; i < e; i++)
@@ -618,6 +620,7 @@ void Engine::evaluate_chunk(int64_t B, const int32_t* occ, double* psi, hipStrea
     std::vector<char> host(size_t(B * es_));
     check_hip(hipMemcpyAsync(host.data(), G, host.size(), hipMemcpyDeviceToHost, st), "psi copy");
     check_hip(hipStreamSynchronize(st), "psi sync");
+    if (profile_) timer_.collect();
     for (int64_t b = 0; b < B; b++)
         psi[b] = dtype_ == 0 ? reinterpret_cast<double*>(host.data())[b] : double(reinterpret_cast<float*>(host.data())[b]);
 }
@@ -666,6 +669,19 @@ int tns_engine_stats(const tns_engine* e, double* values, int n) {
 }
 
 int tns_engine_reset_stats(tns_engine* e) { TNS_TRY({ e->e->reset_stats(); }) }
+
+int tns_engine_set_profile(tns_engine* e, int on) { TNS_TRY({ e->e->set_profile(on != 0); }) }
+
+int tns_engine_kernel_stats(const tns_engine* e, double* values, int n) {
+    TNS_TRY({
+        const auto& t = e->e->timer();
+        for (int k = 0; k < tns::KernelTimer::NKIND; k++) {
+            const double v[3] = {t.acc[k].launches, t.acc[k].ms, t.acc[k].units};
+            for (int j = 0; j < 3; j++)
+                if (3 * k + j < n) values[3 * k + j] = v[j];
+        }
+    })
+}
 
 int tns_engine_free(tns_engine* e) { TNS_TRY({ delete e; }) }
 

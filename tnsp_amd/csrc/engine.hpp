@@ -79,7 +79,12 @@ class Engine {
                   hipStream_t st);
     void set_chunk(int64_t c) { chunk_ = c > 0 ? c : 1; }
     const EngineStats& stats() const { return stats_; }
-    void reset_stats() { stats_ = EngineStats(); }
+    void reset_stats() {
+        stats_ = EngineStats();
+        timer_.reset();
+    }
+    void set_profile(bool on) { profile_ = on; }
+    const KernelTimer& timer() const { return timer_; }
 
    private:
     enum { A_PERSIST = 0, A_ROW0 = 1, A_ROW1 = 2, A_TEMP = 3, A_SVALS = 4, NARENA = 5 };
@@ -113,6 +118,8 @@ class Engine {
     int64_t remain_cut_ = -1;
     double relative_cut_ = 0.0;
     EngineStats stats_;
+    KernelTimer timer_;
+    bool profile_ = false;
     std::vector<double> svals_host_;
 };
 

@@ -43,6 +43,31 @@ char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
     return dev_;
 }
 
+hipEvent_t KernelTimer::get() {
+    if (next == pool.size()) {
+        hipEvent_t e;
+        check_hip(hipEventCreate(&e), "timer event");
+        pool.push_back(e);
+    }
+    return pool[next++];
+}
+
+void KernelTimer::collect() {
+    for (auto& p : pending) {
+        float ms = 0;
+        check_hip(hipEventElapsedTime(&ms, p.a, p.b), "timer elapsed");
+        acc[p.kind].ms += ms;
+        acc[p.kind].units += p.units;
+        acc[p.kind].launches += 1;
+    }
+    pending.clear();
+    next = 0;
+}
+
+KernelTimer::~KernelTimer() {
+    for (auto e : pool) hipEventDestroy(e);
+}
+
 template<typename T>
 size_t Batch::put(const std::vector<T>& v) {
     const size_t off = (blob_.size() + 255) / 256 * 256;
@@ -76,11 +101,15 @@ void Batch::perm_one(const PermTask& t, void* const* bufs, int64_t es) {
     if (t.n <= 0) return;
     pperm_.push_back(bind_perm(t, bufs, es, es));
     perm_elems_ += t.n;
+    pend_units_[PERM] += double(2 * t.n * es);
 }
 
 void Batch::diag(const std::vector<PermTask>& ts, void* const* bufs, int64_t es) {
     for (auto& t : ts)
-        if (t.n > 0) pdiag_.push_back(bind_perm(t, bufs, 8, es));
+        if (t.n > 0) {
+            pdiag_.push_back(bind_perm(t, bufs, 8, es));
+            pend_units_[DIAG] += double(t.n * (8 + es));
+        }
 }
 
 void Batch::flush_diag() {
@@ -90,11 +119,17 @@ void Batch::flush_diag() {
 
 void Batch::fill(const std::vector<FillTask>& ts, void* const* bufs, int64_t es) {
     for (auto& t : ts)
-        if (t.n > 0) pfill_.push_back(FillDev{static_cast<char*>(bufs[t.buf]) + t.off * es, t.n});
+        if (t.n > 0) {
+            pfill_.push_back(FillDev{static_cast<char*>(bufs[t.buf]) + t.off * es, t.n});
+            pend_units_[FILL] += double(t.n * es);
+        }
 }
 
 void Batch::fill_raw(void* p, int64_t n) {
-    if (n > 0) pfill_.push_back(FillDev{p, n});
+    if (n > 0) {
+        pfill_.push_back(FillDev{p, n});
+        pend_units_[FILL] += double(n * (dtype_ == 0 ? 8 : 4));
+    }
 }
 
 void Batch::flush_flat(Kind kind, std::vector<PermDev>& pd, std::vector<FillDev>& fd) {
@@ -121,6 +156,8 @@ void Batch::flush_flat(Kind kind, std::vector<PermDev>& pd, std::vector<FillDev>
     c.wg = put(wg);
     c.count = int64_t(T);
     c.nwg = nwg;
+    c.units = pend_units_[kind];
+    pend_units_[kind] = 0;
     cmds_.push_back(c);
     pd.clear();
     fd.clear();
@@ -152,6 +189,7 @@ void Batch::gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es)
         g.ta = t.ta;
         g.tb = t.tb;
         g.alpha = t.alpha;
+        pend_units_[GEMM] += 2.0 * double(t.m) * double(t.n) * double(t.k);
         const int32_t id = int32_t(pgemm_.size());
         pgemm_.push_back(g);
         const int tm = (t.m + GEMM_TILE - 1) / GEMM_TILE, tn = (t.n + GEMM_TILE - 1) / GEMM_TILE;
@@ -172,6 +210,8 @@ void Batch::flush_gemm() {
     c.wg = 0;
     c.count = int64_t(pgemm_.size());
     c.nwg = int64_t(ptile_.size());
+    c.units = pend_units_[GEMM];
+    pend_units_[GEMM] = 0;
     cmds_.push_back(c);
     pgemm_.clear();
     ptile_.clear();
@@ -190,6 +230,8 @@ void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, dou
         q.lq = t.lq;
         const int64_t mx = t.lq ? t.n : t.m, nx = t.lq ? t.m : t.n, k = std::min(mx, nx);
         cur += mx * nx + mx * k + k;
+        const double M = double(std::max(t.m, t.n)), N = double(std::min(t.m, t.n));
+        pend_units_[QR] += 4.0 * M * N * N - 4.0 * N * N * N / 3.0;
         pqr_.push_back(q);
     }
 }
@@ -201,6 +243,8 @@ void Batch::flush_qr() {
     c.tasks = put(pqr_);
     c.prefix = c.wg = 0;
     c.count = c.nwg = int64_t(pqr_.size());
+    c.units = pend_units_[QR];
+    pend_units_[QR] = 0;
     cmds_.push_back(c);
     pqr_.clear();
 }
@@ -218,6 +262,7 @@ void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, d
         s.n = t.n;
         const int64_t mx = std::max(t.m, t.n), kx = std::min(t.m, t.n);
         cur += mx * kx + kx * kx + kx;
+        pend_units_[SVD] += 14.0 * double(mx) * double(kx) * double(kx) + 8.0 * double(kx) * double(kx) * double(kx);
         psvd_.push_back(s);
     }
 }
@@ -229,6 +274,8 @@ void Batch::flush_svd() {
     c.tasks = put(psvd_);
     c.prefix = c.wg = 0;
     c.count = c.nwg = int64_t(psvd_.size());
+    c.units = pend_units_[SVD];
+    pend_units_[SVD] = 0;
     cmds_.push_back(c);
     psvd_.clear();
 }
@@ -243,6 +290,12 @@ void Batch::run(hipStream_t st) {
     if (cmds_.empty()) return;
     char* base = Staging::get().upload(blob_, st);
     for (auto& c : cmds_) {
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (timer_) {
+            ea = timer_->get();
+            eb = timer_->get();
+            check_hip(hipEventRecord(ea, st), "timer record");
+        }
         switch (c.kind) {
             case PERM:
                 launch_permute(reinterpret_cast<const PermDev*>(base + c.tasks),
@@ -270,6 +323,10 @@ void Batch::run(hipStream_t st) {
                 launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, dtype_, st);
                 break;
         }
+        if (timer_) {
+            check_hip(hipEventRecord(eb, st), "timer record");
+            timer_->add(int(c.kind), ea, eb, c.units);
+        }
     }
     reset();
 }
@@ -285,6 +342,7 @@ void Batch::reset() {
     pqr_.clear();
     psvd_.clear();
     perm_elems_ = 0;
+    for (auto& u : pend_units_) u = 0;
 }
 
 }  // namespace tns

@@ -32,6 +32,32 @@ class Staging {
     bool pending_ = false;
 };
 
+// Per-kind device timing of batched launches with HIP events recorded on the
+// launch stream, plus the algorithmic work of each launch (flops for gemm /
+// qr / svd, bytes moved for permute / fill).
+struct KernelTimer {
+    enum { PERM, FILL, GEMM, QR, SVD, DIAG, NKIND };
+    struct Acc {
+        double ms = 0, units = 0, launches = 0;
+    } acc[NKIND];
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+        double units;
+    };
+    std::vector<hipEvent_t> pool;
+    size_t next = 0;
+    std::vector<Pending> pending;
+    hipEvent_t get();
+    void add(int kind, hipEvent_t a, hipEvent_t b, double units) { pending.push_back({kind, a, b, units}); }
+    // call after the stream has been synchronised
+    void collect();
+    void reset() {
+        for (auto& a : acc) a = Acc();
+    }
+    ~KernelTimer();
+};
+
 class Batch {
    public:
     explicit Batch(int dtype) : dtype_(dtype) {}
@@ -58,6 +84,7 @@ class Batch {
     bool empty() const { return cmds_.empty(); }
     int64_t perm_elements() const { return perm_elems_; }
     void reset();
+    void set_timer(KernelTimer* t) { timer_ = t; }
 
    private:
     enum Kind { PERM, FILL, GEMM, QR, SVD, DIAG };
@@ -65,7 +92,10 @@ class Batch {
         Kind kind;
         size_t tasks, prefix, wg;  // blob offsets
         int64_t count, nwg;
+        double units;  // algorithmic flops or bytes of this launch
     };
+    KernelTimer* timer_ = nullptr;
+    double pend_units_[6] = {0, 0, 0, 0, 0, 0};
     template<typename T>
     size_t put(const std::vector<T>& v);
     void flush_flat(Kind kind, std::vector<PermDev>& pd, std::vector<FillDev>& fd);
