@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=0, help="configurations per lockstep pass (0 = batch)")
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=4, help="concurrent lockstep lanes (host thread + HIP stream each)")
     ap.add_argument("--workload", default="C4", choices=["C2", "C3", "C4"])
     ap.add_argument("--cpu-configs", type=int, default=2, help="configs per host process for the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0)
@@ -120,12 +121,14 @@ def main():
     storages = peps.random_storages(spec)
     lat = peps.Lattice(spec, storages)
     threads = args.threads or int(os.environ.get("TNS_THREADS", min(16, os.cpu_count() or 1)))
-    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk or args.batch)
+    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk or -(-args.batch // args.lanes),
+                          lanes=args.lanes)
     B = args.batch
     total_steps = args.warmup + args.steps
     particles = (24, 40) if spec.kind == "FermiZ2" else None
     configs = peps.sample_configurations(spec, B * total_steps, seed=1234, particles=particles)
-    print(f"[bench] workload {args.workload} batch {B} threads {threads}", file=sys.stderr, flush=True)
+    print(f"[bench] workload {args.workload} batch {B} threads {threads} lanes {args.lanes}", file=sys.stderr,
+          flush=True)
     for w in range(args.warmup):
         t0 = time.time()
         eng.evaluate(configs[w * B:(w + 1) * B])
@@ -177,7 +180,7 @@ def main():
                                f"{list(spec.segments)} around tV charge backgrounds, Dc={spec.cut}, N={spec.N}, "
                                f"{B} configurations per step drawn uniformly at fixed particle number, "
                                f"batched lockstep engine, fp64 throughout",
-                   "batch": B, "threads": threads},
+                   "batch": B, "threads": threads, "lanes": args.lanes},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": {"kernel_ms": {k: v["ms"] for k, v in ks.items()},

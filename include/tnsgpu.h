@@ -100,6 +100,10 @@ typedef struct tns_engine tns_engine;
 int tns_engine_create(int kind, int L1, int L2, const tns_shape* const* sites, const void* const* site_data,
                       int dtype, const int32_t* name_ids, int n_names, int threads, tns_engine** out);
 int tns_engine_set_chunk(tns_engine* e, int64_t chunk);
+/* lanes: independent lockstep pipelines over the same PEPS, each on its own
+ * host thread and HIP stream; configurations are split over them so one
+ * lane's host planning and decomposition tails overlap the others' kernels */
+int tns_engine_set_lanes(tns_engine* e, int lanes);
 int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
                         double relative_cut, double* psi, void* stream);
 /* values: plan_s exec_s total_s gemm_flops qr_flops svd_flops perm_elems

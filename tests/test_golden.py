@@ -122,7 +122,8 @@ def test_amplitude_gpu_plain_and_engine_vs_tat(name):
         v, t = ppeps.amplitude_plain(lat, configs[c])
         assert t.names == psis[c]["names"]
         assert abs(v - ref[c]) <= 1e-10 * abs(ref[c]) + 1e-300
-    eng = AmplitudeEngine(lat, chunk=7)
-    got = eng.evaluate(configs)
-    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-300)
-    assert np.array_equal(np.sign(got), np.sign(ref))
+    for chunk, lanes in ((7, 1), (5, 3)):
+        eng = AmplitudeEngine(lat, chunk=chunk, lanes=lanes)
+        got = eng.evaluate(configs)
+        np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-300)
+        assert np.array_equal(np.sign(got), np.sign(ref))

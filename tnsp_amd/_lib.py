@@ -52,6 +52,7 @@ _SIGNATURES = {
     "tns_engine_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int,
                           ctypes.c_int, c_vpp],
     "tns_engine_set_chunk": [c_vp, ctypes.c_int64],
+    "tns_engine_set_lanes": [c_vp, ctypes.c_int],
     "tns_engine_evaluate": [c_vp, ctypes.c_int64, c_i32p, ctypes.c_int64, ctypes.c_double, c_dblp, c_vp],
     "tns_engine_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_reset_stats": [c_vp],

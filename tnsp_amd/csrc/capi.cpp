@@ -133,14 +133,8 @@ struct Layout {
 };
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-int64_t qr_scratch(const QrTask& t) {
-    const int64_t mx = t.lq ? t.n : t.m, nx = t.lq ? t.m : t.n, k = std::min(mx, nx);
-    return mx * nx + mx * k + k;
-}
-int64_t svd_scratch(const SvdTask& t) {
-    const int64_t mx = std::max(t.m, t.n), kx = std::min(t.m, t.n);
-    return mx * kx + kx * kx + kx;
-}
+int64_t qr_scratch(const QrTask& t) { return qr_doubles(t.m, t.n); }
+int64_t svd_scratch(const SvdTask& t) { return svd_doubles(t.m, t.n); }
 
 Layout layout_of(const tns_plan* p, int dtype) {
     const int64_t es = dtype == 0 ? 8 : 4;

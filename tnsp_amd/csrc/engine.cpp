@@ -5,6 +5,8 @@
 #include <chrono>
 #include <cstring>
 #include <exception>
+#include <memory>
+#include <thread>
 
 #include "../../include/tnsgpu.h"
 #include "handles.hpp"
@@ -104,7 +106,7 @@ void DevArena::ensure(size_t keep, hipStream_t st) {
 }
 
 void DevArena::release() {
-    if (base) hipFree(base);
+    if (base) (void)hipFree(base);
     base = nullptr;
     cap = used = 0;
 }
@@ -135,6 +137,7 @@ void Engine::run_batch(Batch& b, hipStream_t st) {
     const double t0 = now();
     stats_.perm_elems += double(b.perm_elements());
     if (profile_) b.set_timer(&timer_);
+    b.set_staging(&staging_);
     b.run(st);
     stats_.exec_s += now() - t0;
 }
@@ -236,10 +239,7 @@ void Engine::qr(std::vector<Slot*>& t, const std::vector<int32_t>& free_names, i
         t1[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t1.size * es_)));
         t2[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t2.size * es_)));
         int64_t sc = 0;
-        for (auto& task : p.qrs) {
-            const int64_t mx = task.lq ? task.n : task.m, nx = task.lq ? task.m : task.n, k = std::min(mx, nx);
-            sc += mx * nx + mx * k + k;
-        }
+        for (auto& task : p.qrs) sc += qr_doubles(task.m, task.n);
         ts[i] = int64_t(arenas_[A_TEMP].alloc(size_t(sc * 8)));
         oq[i] = int64_t(arenas_[oa].alloc(size_t(p.q.size * es_)));
         orr[i] = int64_t(arenas_[oa].alloc(size_t(p.r.size * es_)));
@@ -300,10 +300,7 @@ void Engine::svd(std::vector<Slot*>& t, const std::vector<int32_t>& free_u, int3
         t1[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t1.size * es_)));
         t2[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t2.size * es_)));
         int64_t sc = 0;
-        for (auto& task : p.svds) {
-            const int64_t mx = std::max(task.m, task.n), kx = std::min(task.m, task.n);
-            sc += mx * kx + kx * kx + kx;
-        }
+        for (auto& task : p.svds) sc += svd_doubles(task.m, task.n);
         ts[i] = int64_t(arenas_[A_TEMP].alloc(size_t(sc * 8)));
         sv_off[i] = int64_t(arenas_[A_SVALS].used);
         arenas_[A_SVALS].used += size_t(p.s_total * 8);  // contiguous, no alignment gaps
@@ -628,8 +625,35 @@ void Engine::evaluate_chunk(int64_t B, const int32_t* occ, double* psi, hipStrea
 }  // namespace tns
 
 // ------------------------------------------------------------------ C ABI
+// A handle holds one or more lanes: independent engines over the same PEPS,
+// each with its own host thread, HIP stream, arenas and task staging.  A call
+// splits the configurations over the lanes so that the decomposition tail
+// and host planning of one lane overlap the kernels of the others.
 struct tns_engine {
-    std::unique_ptr<tns::Engine> e;
+    int kind, L1, L2, dtype, threads;
+    std::vector<tns::Shape> sites;
+    std::vector<const void*> data;
+    std::vector<int32_t> names;
+    int64_t chunk = 1024;
+    bool profile = false;
+    std::vector<std::unique_ptr<tns::Engine>> lanes;
+    std::vector<hipStream_t> streams;  // streams[0] unused (caller's stream)
+    ~tns_engine() {
+        lanes.clear();
+        for (size_t i = 1; i < streams.size(); i++)
+            if (streams[i]) (void)hipStreamDestroy(streams[i]);
+    }
+    void add_lanes(int n) {
+        const int per = std::max(1, threads / std::max(1, n));
+        while (int(lanes.size()) < n) {
+            lanes.push_back(std::make_unique<tns::Engine>(kind, L1, L2, sites, data, dtype, names, per));
+            lanes.back()->set_chunk(chunk);
+            lanes.back()->set_profile(profile);
+            hipStream_t s = nullptr;
+            if (lanes.size() > 1) tns::check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
+            streams.push_back(s);
+        }
+    }
 };
 
 extern "C" {
@@ -638,48 +662,117 @@ int tns_engine_create(int kind, int L1, int L2, const tns_shape* const* sites, c
                       int dtype, const int32_t* name_ids, int n_names, int threads, tns_engine** out) {
     TNS_TRY({
         if (dtype != 0 && dtype != 1) throw tns::Error("unsupported dtype");
-        std::vector<tns::Shape> sh;
-        std::vector<const void*> dat;
+        auto h = std::make_unique<tns_engine>();
+        h->kind = kind;
+        h->L1 = L1;
+        h->L2 = L2;
+        h->dtype = dtype;
+        h->threads = std::max(1, threads);
         for (int i = 0; i < L1 * L2; i++) {
             if (sites[i]->s.kind_id != kind) throw tns::Error("site symmetry differs from engine kind");
-            sh.push_back(sites[i]->s);
-            dat.push_back(site_data[i]);
+            h->sites.push_back(sites[i]->s);
+            h->data.push_back(site_data[i]);
         }
-        std::vector<int32_t> names(name_ids, name_ids + n_names);
-        auto* h = new tns_engine;
-        h->e = std::make_unique<tns::Engine>(kind, L1, L2, sh, dat, dtype, names, threads);
-        *out = h;
+        h->names.assign(name_ids, name_ids + n_names);
+        h->add_lanes(1);
+        *out = h.release();
     })
 }
 
-int tns_engine_set_chunk(tns_engine* e, int64_t chunk) { TNS_TRY({ e->e->set_chunk(chunk); }) }
+int tns_engine_set_chunk(tns_engine* e, int64_t chunk) {
+    TNS_TRY({
+        e->chunk = chunk;
+        for (auto& l : e->lanes) l->set_chunk(chunk);
+    })
+}
+
+int tns_engine_set_lanes(tns_engine* e, int lanes) {
+    TNS_TRY({
+        if (lanes < 1 || lanes > 16) throw tns::Error("lanes must be in [1, 16]");
+        if (lanes < int(e->lanes.size())) {
+            e->lanes.resize(size_t(lanes));
+            for (size_t i = size_t(lanes); i < e->streams.size(); i++)
+                if (e->streams[i]) (void)hipStreamDestroy(e->streams[i]);
+            e->streams.resize(size_t(lanes));
+        }
+        e->add_lanes(lanes);
+    })
+}
 
 int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
                         double relative_cut, double* psi, void* stream) {
-    TNS_TRY({ e->e->evaluate(nconf, occupations, remain_cut, relative_cut, psi, static_cast<hipStream_t>(stream)); })
+    TNS_TRY({
+        hipStream_t caller = static_cast<hipStream_t>(stream);
+        const int64_t K = std::min<int64_t>(int64_t(e->lanes.size()), std::max<int64_t>(1, nconf));
+        if (K <= 1) {
+            e->lanes[0]->evaluate(nconf, occupations, remain_cut, relative_cut, psi, caller);
+        } else {
+            // lanes run on their own streams: order them after the caller's work
+            tns::check_hip(hipStreamSynchronize(caller), "caller stream sync");
+            const int64_t per = int64_t(e->L1) * e->L2;
+            std::vector<std::exception_ptr> errs;
+            errs.resize(size_t(K));
+            auto run = [&](int64_t k) {
+                const int64_t b = nconf * k / K, en = nconf * (k + 1) / K;
+                try {
+                    e->lanes[k]->evaluate(en - b, occupations + b * per, remain_cut, relative_cut, psi + b,
+                                          k == 0 ? caller : e->streams[k]);
+                } catch (...) {
+                    errs[k] = std::current_exception();
+                }
+            };
+            std::vector<std::thread> ts;
+            for (int64_t k = 1; k < K; k++) ts.emplace_back(run, k);
+            run(0);
+            for (auto& t : ts) t.join();
+            for (auto& ep : errs)
+                if (ep) std::rethrow_exception(ep);
+        }
+    })
 }
 
 int tns_engine_stats(const tns_engine* e, double* values, int n) {
     TNS_TRY({
-        const auto& s = e->e->stats();
-        const double v[11] = {s.plan_s,    s.exec_s,     s.total_s,  s.gemm_flops, s.qr_flops, s.svd_flops,
-                              s.perm_elems, s.launches, s.peak_bytes, s.configs,   0.0};
+        double v[11] = {0};
+        for (auto& l : e->lanes) {
+            const auto& s = l->stats();
+            const double x[11] = {s.plan_s,     s.exec_s,   s.total_s,    s.gemm_flops, s.qr_flops, s.svd_flops,
+                                  s.perm_elems, s.launches, s.peak_bytes, s.configs,    0.0};
+            for (int i = 0; i < 11; i++) v[i] += x[i];
+        }
+        // wall-clock style entries: the lanes run concurrently
+        double tmax = 0;
+        for (auto& l : e->lanes) tmax = std::max(tmax, l->stats().total_s);
+        v[2] = tmax;
         for (int i = 0; i < n && i < 11; i++) values[i] = v[i];
     })
 }
 
-int tns_engine_reset_stats(tns_engine* e) { TNS_TRY({ e->e->reset_stats(); }) }
+int tns_engine_reset_stats(tns_engine* e) {
+    TNS_TRY({
+        for (auto& l : e->lanes) l->reset_stats();
+    })
+}
 
-int tns_engine_set_profile(tns_engine* e, int on) { TNS_TRY({ e->e->set_profile(on != 0); }) }
+int tns_engine_set_profile(tns_engine* e, int on) {
+    TNS_TRY({
+        e->profile = on != 0;
+        for (auto& l : e->lanes) l->set_profile(on != 0);
+    })
+}
 
 int tns_engine_kernel_stats(const tns_engine* e, double* values, int n) {
     TNS_TRY({
-        const auto& t = e->e->timer();
-        for (int k = 0; k < tns::KernelTimer::NKIND; k++) {
-            const double v[3] = {t.acc[k].launches, t.acc[k].ms, t.acc[k].units};
-            for (int j = 0; j < 3; j++)
-                if (3 * k + j < n) values[3 * k + j] = v[j];
+        std::vector<double> acc(size_t(3 * tns::KernelTimer::NKIND), 0.0);
+        for (auto& l : e->lanes) {
+            const auto& t = l->timer();
+            for (int k = 0; k < tns::KernelTimer::NKIND; k++) {
+                acc[3 * k] += t.acc[k].launches;
+                acc[3 * k + 1] += t.acc[k].ms;
+                acc[3 * k + 2] += t.acc[k].units;
+            }
         }
+        for (int i = 0; i < n && i < int(acc.size()); i++) values[i] = acc[size_t(i)];
     })
 }
 

@@ -118,6 +118,7 @@ class Engine {
     int64_t remain_cut_ = -1;
     double relative_cut_ = 0.0;
     EngineStats stats_;
+    Staging staging_;
     KernelTimer timer_;
     bool profile_ = false;
     std::vector<double> svals_host_;

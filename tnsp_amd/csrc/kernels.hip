@@ -169,6 +169,14 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// butterfly sum within aligned groups of G lanes (G = 16, 32 or 64): every
+// lane of the group ends with the same, order-independent value
+__device__ inline double group_sum(double v, int G) {
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+template<int NT>
 __device__ inline double block_sum(double v, double* red) {
     v = wave_sum(v);
     const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
@@ -176,51 +184,51 @@ __device__ inline double block_sum(double v, double* red) {
     if (lane == 0) red[wave] = v;
     __syncthreads();
     double t = 0;
-    for (int w = 0; w < WG / WAVE; w++) t += red[w];
+    for (int w = 0; w < NT / WAVE; w++) t += red[w];
     return t;
 }
 
+// lanes per group for work along a column of length len
+__device__ inline int group_lanes(int len) { return len <= 48 ? 16 : (len <= 96 ? 32 : 64); }
+
 // --------------------------------------------------------------------- qr
-// Householder QR of X (mx x nx, column-major copy in scratch), X = M (qr) or
-// X = M^T (lq).  Explicit Q by backward accumulation (as orgqr does).
+// Householder QR of X (mx x nx, column-major), X = M (qr) or X = M^T (lq);
+// R is written out first, then Q is formed in place from the reflectors
+// (backward accumulation, as LAPACK's orgqr does).  X lives in LDS when it
+// fits, otherwise in global scratch.
 template<typename T>
-__global__ __launch_bounds__(WG) void qr_kernel(const QrDev* __restrict__ tasks) {
-    __shared__ double red[WG / WAVE];
-    __shared__ double sh_tau, sh_beta;
-    const QrDev tk = tasks[blockIdx.x];
+__device__ __forceinline__ void qr_body(const QrDev& tk, double* X, double* tau, double* red, double* bc) {
     const int m = tk.m, n = tk.n, lq = tk.lq;
     const int mx = lq ? n : m, nx = lq ? m : n, k = mx < nx ? mx : nx;
-    double* W = tk.work;
-    double* Q = W + int64_t(mx) * nx;
-    double* tau = Q + int64_t(mx) * k;
     const T* in = static_cast<const T*>(tk.in);
-    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE, nw = WG / WAVE;
-    for (int64_t e = tid; e < int64_t(mx) * nx; e += WG) {
+    const int tid = threadIdx.x;
+    const int G = group_lanes(mx), NG = QR_WG / G, g = tid / G, gl = tid % G;
+    for (int64_t e = tid; e < int64_t(mx) * nx; e += QR_WG) {
         const int r = int(e % mx), c = int(e / mx);
-        W[e] = double(lq ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
+        X[e] = double(lq ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
     }
     __syncthreads();
     for (int j = 0; j < k; j++) {
-        double* cj = W + int64_t(j) * mx;
+        double* cj = X + int64_t(j) * mx;
         double s = 0;
-        for (int r = j + 1 + tid; r < mx; r += WG) s += cj[r] * cj[r];
-        const double sigma = block_sum(s, red);
+        for (int r = j + 1 + tid; r < mx; r += QR_WG) s += cj[r] * cj[r];
+        const double sigma = block_sum<QR_WG>(s, red);
         if (tid == 0) {
             const double alpha = cj[j];
             if (sigma == 0.0) {
-                sh_tau = 0.0;
-                sh_beta = alpha;
+                bc[0] = 0.0;
+                bc[1] = alpha;
             } else {
                 const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-                sh_tau = (beta - alpha) / beta;
-                sh_beta = beta;
+                bc[0] = (beta - alpha) / beta;
+                bc[1] = beta;
             }
         }
         __syncthreads();
-        const double tj = sh_tau, beta = sh_beta;
+        const double tj = bc[0], beta = bc[1];
         if (tj != 0.0) {
             const double scale = 1.0 / (cj[j] - beta);
-            for (int r = j + 1 + tid; r < mx; r += WG) cj[r] *= scale;
+            for (int r = j + 1 + tid; r < mx; r += QR_WG) cj[r] *= scale;
         }
         __syncthreads();
         if (tid == 0) {
@@ -228,95 +236,122 @@ __global__ __launch_bounds__(WG) void qr_kernel(const QrDev* __restrict__ tasks)
             tau[j] = tj;
         }
         if (tj != 0.0) {
-            for (int c = j + 1 + wave; c < nx; c += nw) {
-                double* cc = W + int64_t(c) * mx;
+            for (int c = j + 1 + g; c < nx; c += NG) {
+                double* cc = X + int64_t(c) * mx;
                 double d = 0;
-                for (int r = j + 1 + lane; r < mx; r += WAVE) d += cj[r] * cc[r];
-                d = wave_sum(d) + cc[j];
+                for (int r = j + 1 + gl; r < mx; r += G) d += cj[r] * cc[r];
+                d = group_sum(d, G) + cc[j];
                 const double f = tj * d;
-                if (lane == 0) cc[j] -= f;
-                for (int r = j + 1 + lane; r < mx; r += WAVE) cc[r] -= f * cj[r];
-            }
-        }
-        __syncthreads();
-    }
-    // Q = H_0 ... H_{k-1} [I; 0]
-    for (int64_t e = tid; e < int64_t(mx) * k; e += WG) {
-        const int r = int(e % mx), c = int(e / mx);
-        Q[e] = (r == c) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    for (int j = k - 1; j >= 0; j--) {
-        const double tj = tau[j];
-        if (tj != 0.0) {
-            const double* v = W + int64_t(j) * mx;
-            for (int c = j + wave; c < k; c += nw) {
-                double* qc = Q + int64_t(c) * mx;
-                double d = 0;
-                for (int r = j + 1 + lane; r < mx; r += WAVE) d += v[r] * qc[r];
-                d = wave_sum(d) + qc[j];
-                const double f = tj * d;
-                if (lane == 0) qc[j] -= f;
-                for (int r = j + 1 + lane; r < mx; r += WAVE) qc[r] -= f * v[r];
+                if (gl == 0) cc[j] -= f;
+                for (int r = j + 1 + gl; r < mx; r += G) cc[r] -= f * cj[r];
             }
         }
         __syncthreads();
     }
     T* f1 = static_cast<T*>(tk.f1);
     T* f2 = static_cast<T*>(tk.f2);
+    // R (k x nx, upper trapezoid)
     if (!lq) {
-        for (int64_t e = tid; e < int64_t(m) * k; e += WG) {
-            const int r = int(e / k), c = int(e % k);
-            f1[e] = T(Q[int64_t(c) * mx + r]);
-        }
-        for (int64_t e = tid; e < int64_t(k) * n; e += WG) {
+        for (int64_t e = tid; e < int64_t(k) * n; e += QR_WG) {
             const int r = int(e / n), c = int(e % n);
-            f2[e] = T(r <= c ? W[int64_t(c) * mx + r] : 0.0);
+            f2[e] = T(r <= c ? X[int64_t(c) * mx + r] : 0.0);
         }
     } else {
-        // L = R'^T (m x k), Q = Q'^T (k x n)
-        for (int64_t e = tid; e < int64_t(m) * k; e += WG) {
+        for (int64_t e = tid; e < int64_t(m) * k; e += QR_WG) {
             const int i = int(e / k), r = int(e % k);
-            f1[e] = T(r <= i ? W[int64_t(i) * mx + r] : 0.0);
+            f1[e] = T(r <= i ? X[int64_t(i) * mx + r] : 0.0);
         }
-        for (int64_t e = tid; e < int64_t(k) * n; e += WG) {
+    }
+    __syncthreads();
+    for (int j = k - 1; j >= 0; j--) {
+        const double tj = tau[j];
+        const double* v = X + int64_t(j) * mx;
+        if (tj != 0.0) {
+            for (int c = j + 1 + g; c < k; c += NG) {
+                double* qc = X + int64_t(c) * mx;
+                double d = 0;
+                for (int r = j + 1 + gl; r < mx; r += G) d += v[r] * qc[r];
+                d = group_sum(d, G) + qc[j];
+                const double f = tj * d;
+                if (gl == 0) qc[j] -= f;
+                for (int r = j + 1 + gl; r < mx; r += G) qc[r] -= f * v[r];
+            }
+        }
+        __syncthreads();
+        double* cj = X + int64_t(j) * mx;
+        for (int r = tid; r < mx; r += QR_WG) {
+            if (r > j)
+                cj[r] *= -tj;
+            else if (r == j)
+                cj[r] = 1.0 - tj;
+            else
+                cj[r] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (!lq) {
+        for (int64_t e = tid; e < int64_t(m) * k; e += QR_WG) {
+            const int r = int(e / k), c = int(e % k);
+            f1[e] = T(X[int64_t(c) * mx + r]);
+        }
+    } else {
+        for (int64_t e = tid; e < int64_t(k) * n; e += QR_WG) {
             const int r = int(e / n), i = int(e % n);
-            f2[e] = T(Q[int64_t(r) * mx + i]);
+            f2[e] = T(X[int64_t(r) * mx + i]);
         }
     }
 }
 
-// -------------------------------------------------------------------- svd
-// One-sided Jacobi (Hestenes) on the kx = min(m, n) columns of X, X = M when
-// m >= n, X = M^T otherwise; round-robin pair ordering, one pair per wave.
 template<typename T>
-__global__ __launch_bounds__(WG) void svd_kernel(const SvdDev* __restrict__ tasks) {
-    __shared__ int rotated;
-    const SvdDev tk = tasks[blockIdx.x];
+__global__ __launch_bounds__(QR_WG) void qr_kernel_lds(const QrDev* __restrict__ tasks) {
+    extern __shared__ double smem[];
+    __shared__ double red[QR_WG / WAVE];
+    __shared__ double bc[2];
+    const QrDev tk = tasks[blockIdx.x];
+    const int mx = tk.lq ? tk.n : tk.m, nx = tk.lq ? tk.m : tk.n;
+    qr_body<T>(tk, smem, smem + int64_t(mx) * nx, red, bc);
+}
+
+template<typename T>
+__global__ __launch_bounds__(QR_WG) void qr_kernel_glb(const QrDev* __restrict__ tasks) {
+    __shared__ double red[QR_WG / WAVE];
+    __shared__ double bc[2];
+    const QrDev tk = tasks[blockIdx.x];
+    const int mx = tk.lq ? tk.n : tk.m, nx = tk.lq ? tk.m : tk.n;
+    qr_body<T>(tk, tk.work, tk.work + int64_t(mx) * nx, red, bc);
+}
+
+// -------------------------------------------------------------------- svd
+// One-sided Jacobi (Hestenes) on the kx = min(m, n) columns of X (X = M when
+// m >= n, X = M^T otherwise), round-robin pair ordering; each pair is handled
+// by one group of G lanes.  X, V (kx x kx) and sigma live in LDS when they
+// fit, otherwise in global scratch.  Singular values sorted descending.
+template<typename T>
+__device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V, double* sig, int* flag) {
     const int m = tk.m, n = tk.n;
     const bool trans = m < n;
     const int mx = trans ? n : m, kx = trans ? m : n;
-    double* X = tk.work;
-    double* V = X + int64_t(mx) * kx;
-    double* sig = V + int64_t(kx) * kx;
     const T* in = static_cast<const T*>(tk.in);
-    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE, nw = WG / WAVE;
-    for (int64_t e = tid; e < int64_t(mx) * kx; e += WG) {
+    const int tid = threadIdx.x;
+    const int G = group_lanes(mx), NG = SVD_WG / G, g = tid / G, gl = tid % G;
+    for (int64_t e = tid; e < int64_t(mx) * kx; e += SVD_WG) {
         const int r = int(e % mx), c = int(e / mx);
         X[e] = double(trans ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
     }
-    for (int64_t e = tid; e < int64_t(kx) * kx; e += WG) {
+    for (int64_t e = tid; e < int64_t(kx) * kx; e += SVD_WG) {
         const int r = int(e % kx), c = int(e / kx);
         V[e] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
     const int P = kx + (kx & 1);
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
+    constexpr int RPT = 4;
+    const bool cached = mx <= RPT * G;
     for (int sweep = 0; sweep < 60 && kx > 1; sweep++) {
-        if (tid == 0) rotated = 0;
+        if (tid == 0) *flag = 0;
         __syncthreads();
         for (int round = 0; round < P - 1; round++) {
-            for (int pi = wave; pi < P / 2; pi += nw) {
+            for (int pi = g; pi < P / 2; pi += NG) {
                 int a = (pi == 0) ? 0 : 1 + (round + pi - 1) % (P - 1);
                 int b = 1 + (round + P - 2 - pi) % (P - 1);
                 if (a >= kx || b >= kx) continue;
@@ -327,77 +362,104 @@ __global__ __launch_bounds__(WG) void svd_kernel(const SvdDev* __restrict__ task
                 }
                 double* xa = X + int64_t(a) * mx;
                 double* xb = X + int64_t(b) * mx;
+                // the pair's columns stay in registers between the dot
+                // products and the rotation (one read, one write per element)
+                double ra[RPT], rb[RPT];
                 double saa = 0, sbb = 0, sab = 0;
-                for (int r = lane; r < mx; r += WAVE) {
-                    const double u = xa[r], w = xb[r];
-                    saa += u * u;
-                    sbb += w * w;
-                    sab += u * w;
+                if (cached) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        const int r = gl + i * G;
+                        const double u = r < mx ? xa[r] : 0.0, w = r < mx ? xb[r] : 0.0;
+                        ra[i] = u;
+                        rb[i] = w;
+                        saa += u * u;
+                        sbb += w * w;
+                        sab += u * w;
+                    }
+                } else {
+                    for (int r = gl; r < mx; r += G) {
+                        const double u = xa[r], w = xb[r];
+                        saa += u * u;
+                        sbb += w * w;
+                        sab += u * w;
+                    }
                 }
-                saa = wave_sum(saa);
-                sbb = wave_sum(sbb);
-                sab = wave_sum(sab);
+                saa = group_sum(saa, G);
+                sbb = group_sum(sbb, G);
+                sab = group_sum(sab, G);
                 if (fabs(sab) > tol * sqrt(saa * sbb) && sab != 0.0) {
                     const double zeta = (sbb - saa) / (2.0 * sab);
                     const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double cs = 1.0 / sqrt(1.0 + t * t);
                     const double sn = cs * t;
-                    for (int r = lane; r < mx; r += WAVE) {
-                        const double u = xa[r], w = xb[r];
-                        xa[r] = cs * u - sn * w;
-                        xb[r] = sn * u + cs * w;
+                    if (cached) {
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) {
+                            const int r = gl + i * G;
+                            if (r < mx) {
+                                xa[r] = cs * ra[i] - sn * rb[i];
+                                xb[r] = sn * ra[i] + cs * rb[i];
+                            }
+                        }
+                    } else {
+                        for (int r = gl; r < mx; r += G) {
+                            const double u = xa[r], w = xb[r];
+                            xa[r] = cs * u - sn * w;
+                            xb[r] = sn * u + cs * w;
+                        }
                     }
                     double* va = V + int64_t(a) * kx;
                     double* vb = V + int64_t(b) * kx;
-                    for (int r = lane; r < kx; r += WAVE) {
+                    for (int r = gl; r < kx; r += G) {
                         const double u = va[r], w = vb[r];
                         va[r] = cs * u - sn * w;
                         vb[r] = sn * u + cs * w;
                     }
-                    if (lane == 0) rotated = 1;
+                    if (gl == 0) *flag = 1;
                 }
             }
             __syncthreads();
         }
-        const int any = rotated;
+        const int any = *flag;
         __syncthreads();
         if (!any) break;
     }
-    for (int c = wave; c < kx; c += nw) {
+    for (int c = g; c < kx; c += NG) {
         double* xc = X + int64_t(c) * mx;
         double s = 0;
-        for (int r = lane; r < mx; r += WAVE) s += xc[r] * xc[r];
-        s = wave_sum(s);
+        for (int r = gl; r < mx; r += G) s += xc[r] * xc[r];
+        s = group_sum(s, G);
         const double sc = sqrt(s);
         const double inv = sc > 0 ? 1.0 / sc : 0.0;
-        for (int r = lane; r < mx; r += WAVE) xc[r] *= inv;
-        if (lane == 0) sig[c] = sc;
+        for (int r = gl; r < mx; r += G) xc[r] *= inv;
+        if (gl == 0) sig[c] = sc;
     }
     __syncthreads();
     // exactly rank-deficient blocks: complete the zero columns to an
     // orthonormal set (LAPACK's gesvd returns orthonormal U and V^T too)
-    if (wave == 0) {
+    if (g == 0) {
         for (int c = 0; c < kx; c++) {
             if (sig[c] > 0.0) continue;
             double* xc = X + int64_t(c) * mx;
             for (int j = 0; j < mx; j++) {
-                for (int r = lane; r < mx; r += WAVE) xc[r] = (r == j) ? 1.0 : 0.0;
+                for (int r = gl; r < mx; r += G) xc[r] = (r == j) ? 1.0 : 0.0;
                 for (int pass = 0; pass < 2; pass++) {
                     for (int d = 0; d < kx; d++) {
                         if (d == c || (sig[d] == 0.0 && d > c)) continue;
                         const double* xd = X + int64_t(d) * mx;
                         double dot = 0;
-                        for (int r = lane; r < mx; r += WAVE) dot += xc[r] * xd[r];
-                        dot = wave_sum(dot);
-                        for (int r = lane; r < mx; r += WAVE) xc[r] -= dot * xd[r];
+                        for (int r = gl; r < mx; r += G) dot += xc[r] * xd[r];
+                        dot = group_sum(dot, G);
+                        for (int r = gl; r < mx; r += G) xc[r] -= dot * xd[r];
                     }
                 }
                 double nn = 0;
-                for (int r = lane; r < mx; r += WAVE) nn += xc[r] * xc[r];
-                nn = wave_sum(nn);
+                for (int r = gl; r < mx; r += G) nn += xc[r] * xc[r];
+                nn = group_sum(nn, G);
                 if (nn > 0.25) {
                     const double inv = 1.0 / sqrt(nn);
-                    for (int r = lane; r < mx; r += WAVE) xc[r] *= inv;
+                    for (int r = gl; r < mx; r += G) xc[r] *= inv;
                     break;
                 }
             }
@@ -406,26 +468,44 @@ __global__ __launch_bounds__(WG) void svd_kernel(const SvdDev* __restrict__ task
     __syncthreads();
     T* U = static_cast<T*>(tk.u);
     T* Vo = static_cast<T*>(tk.v);
-    for (int c = wave; c < kx; c += nw) {
+    for (int c = g; c < kx; c += NG) {
         const double sc = sig[c];
         int rank = 0;
         for (int d = 0; d < kx; d++) {
             const double sd = sig[d];
             rank += (sd > sc) || (sd == sc && d < c);
         }
-        if (lane == 0) tk.s[rank] = sc;
+        if (gl == 0) tk.s[rank] = sc;
         const double* xc = X + int64_t(c) * mx;
         const double* vc = V + int64_t(c) * kx;
         if (!trans) {
             // M = X diag(sig) V^T with X orthonormal: U (m x k), Vo = V^T
-            for (int r = lane; r < m; r += WAVE) U[int64_t(r) * kx + rank] = T(xc[r]);
-            for (int i = lane; i < n; i += WAVE) Vo[int64_t(rank) * n + i] = T(vc[i]);
+            for (int r = gl; r < m; r += G) U[int64_t(r) * kx + rank] = T(xc[r]);
+            for (int i = gl; i < n; i += G) Vo[int64_t(rank) * n + i] = T(vc[i]);
         } else {
             // M^T = X diag(sig) V^T  =>  M = V diag(sig) X^T
-            for (int i = lane; i < m; i += WAVE) U[int64_t(i) * kx + rank] = T(vc[i]);
-            for (int r = lane; r < n; r += WAVE) Vo[int64_t(rank) * n + r] = T(xc[r]);
+            for (int i = gl; i < m; i += G) U[int64_t(i) * kx + rank] = T(vc[i]);
+            for (int r = gl; r < n; r += G) Vo[int64_t(rank) * n + r] = T(xc[r]);
         }
     }
+}
+
+template<typename T>
+__global__ __launch_bounds__(SVD_WG) void svd_kernel_lds(const SvdDev* __restrict__ tasks) {
+    extern __shared__ double smem[];
+    __shared__ int flag;
+    const SvdDev tk = tasks[blockIdx.x];
+    const int mx = tk.m < tk.n ? tk.n : tk.m, kx = tk.m < tk.n ? tk.m : tk.n;
+    svd_body<T>(tk, smem, smem + int64_t(mx) * kx, smem + int64_t(mx) * kx + int64_t(kx) * kx, &flag);
+}
+
+template<typename T>
+__global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restrict__ tasks) {
+    __shared__ int flag;
+    const SvdDev tk = tasks[blockIdx.x];
+    const int mx = tk.m < tk.n ? tk.n : tk.m, kx = tk.m < tk.n ? tk.m : tk.n;
+    double* X = tk.work;
+    svd_body<T>(tk, X, X + int64_t(mx) * kx, X + int64_t(mx) * kx + int64_t(kx) * kx, &flag);
 }
 
 inline void check(hipError_t e, const char* what) {
@@ -477,21 +557,54 @@ void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, 
     check(hipGetLastError(), "gemm launch");
 }
 
-void launch_qr(const QrDev* tasks, int64_t num, int dtype, hipStream_t stream) {
+namespace {
+template<typename K>
+void allow_lds(K kernel, int64_t bytes) {
+    static int64_t granted[4] = {0, 0, 0, 0};
+    (void)granted;
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(bytes));
+}
+}  // namespace
+
+void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (dtype == 0)
-        hipLaunchKernelGGL(qr_kernel<double>, dim3(unsigned(num)), dim3(WG), 0, stream, tasks);
-    else
-        hipLaunchKernelGGL(qr_kernel<float>, dim3(unsigned(num)), dim3(WG), 0, stream, tasks);
+    if (lds_bytes > 0) {
+        if (dtype == 0) {
+            allow_lds(qr_kernel_lds<double>, lds_bytes);
+            hipLaunchKernelGGL(qr_kernel_lds<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(lds_bytes), stream, tasks);
+        } else {
+            allow_lds(qr_kernel_lds<float>, lds_bytes);
+            hipLaunchKernelGGL(qr_kernel_lds<float>, dim3(unsigned(num)), dim3(QR_WG), size_t(lds_bytes), stream, tasks);
+        }
+    } else {
+        if (dtype == 0)
+            hipLaunchKernelGGL(qr_kernel_glb<double>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
+        else
+            hipLaunchKernelGGL(qr_kernel_glb<float>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
+    }
     check(hipGetLastError(), "qr launch");
 }
 
-void launch_svd(const SvdDev* tasks, int64_t num, int dtype, hipStream_t stream) {
+void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (dtype == 0)
-        hipLaunchKernelGGL(svd_kernel<double>, dim3(unsigned(num)), dim3(WG), 0, stream, tasks);
-    else
-        hipLaunchKernelGGL(svd_kernel<float>, dim3(unsigned(num)), dim3(WG), 0, stream, tasks);
+    if (lds_bytes > 0) {
+        if (dtype == 0) {
+            allow_lds(svd_kernel_lds<double>, lds_bytes);
+            hipLaunchKernelGGL(svd_kernel_lds<double>, dim3(unsigned(num)), dim3(SVD_WG), size_t(lds_bytes), stream,
+                               tasks);
+        } else {
+            allow_lds(svd_kernel_lds<float>, lds_bytes);
+            hipLaunchKernelGGL(svd_kernel_lds<float>, dim3(unsigned(num)), dim3(SVD_WG), size_t(lds_bytes), stream,
+                               tasks);
+        }
+    } else {
+        if (dtype == 0)
+            hipLaunchKernelGGL(svd_kernel_glb<double>, dim3(unsigned(num)), dim3(SVD_WG), 0, stream, tasks);
+        else
+            hipLaunchKernelGGL(svd_kernel_glb<float>, dim3(unsigned(num)), dim3(SVD_WG), 0, stream, tasks);
+    }
     check(hipGetLastError(), "svd launch");
 }
 

@@ -42,7 +42,7 @@ struct QrDev {
     const void* in;
     void* f1;
     void* f2;
-    double* work;  // scratch: mx*nx + mx*k + k doubles
+    double* work;  // global scratch (mx*nx + k doubles) when not LDS-resident
     int32_t m, n, lq, pad;
 };
 
@@ -51,7 +51,7 @@ struct SvdDev {
     void* u;
     double* s;  // singular values are always stored in double
     void* v;
-    double* work;  // scratch: mx*kx + kx*kx + 2*kx doubles
+    double* work;  // global scratch (mx*kx + kx*kx + kx doubles) when not LDS-resident
     int32_t m, n, pad0, pad1;
 };
 
@@ -67,7 +67,26 @@ void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_
                  hipStream_t stream);
 constexpr int GEMM_TILE = 32;
 void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream);
-void launch_qr(const QrDev* tasks, int64_t num, int dtype, hipStream_t stream);
-void launch_svd(const SvdDev* tasks, int64_t num, int dtype, hipStream_t stream);
+// QR / SVD: one workgroup per block.  lds_bytes > 0 selects the LDS-resident
+// kernel with that much dynamic LDS (every task of the launch must fit);
+// 0 selects the kernel working in each task's global scratch.
+constexpr int QR_WG = 512;
+constexpr int SVD_WG = 256;
+// largest LDS working set per workgroup (gfx950: 160 KiB per CU)
+// LDS size classes: a launch reserves the largest working set among its
+// tasks, so tasks are grouped by class (3, 2 or 1 workgroups per CU)
+constexpr int64_t LDS_CLASS[3] = {48 * 1024, 76 * 1024, 152 * 1024};
+constexpr int64_t QR_LDS_MAX = LDS_CLASS[2];
+constexpr int64_t SVD_LDS_MAX = LDS_CLASS[2];
+inline int64_t qr_doubles(int64_t m, int64_t n) {
+    const int64_t k = m < n ? m : n;
+    return m * n + k;
+}
+inline int64_t svd_doubles(int64_t m, int64_t n) {
+    const int64_t mx = m > n ? m : n, kx = m < n ? m : n;
+    return mx * kx + kx * kx + kx;
+}
+void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream);
+void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream);
 
 }  // namespace tns

@@ -1,6 +1,7 @@
 // Launch batching and staging of task arrays (see runtime.hpp).
 #include "runtime.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 
@@ -13,6 +14,13 @@ void check_hip(hipError_t e, const char* what) {
 Staging& Staging::get() {
     static Staging s;
     return s;
+}
+
+Staging::~Staging() {
+    if (pending_ && ev_) hipEventSynchronize(ev_);
+    if (ev_) hipEventDestroy(ev_);
+    if (host_) hipHostFree(host_);
+    if (dev_) hipFree(dev_);
 }
 
 char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
@@ -30,8 +38,9 @@ char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
         check_hip(hipHostMalloc(reinterpret_cast<void**>(&host_), host_cap_, hipHostMallocDefault), "host alloc");
     }
     if (n > dev_cap_) {
-        // kernels queued earlier may still read the old task buffer
-        check_hip(hipDeviceSynchronize(), "device synchronize");
+        // kernels queued earlier may still read the old task buffer; the
+        // process-wide instance may have fed more than one stream
+        check_hip(this == &Staging::get() ? hipDeviceSynchronize() : hipStreamSynchronize(st), "staging synchronize");
         if (dev_) check_hip(hipFree(dev_), "device free");
         dev_cap_ = std::max(n, dev_cap_ * 2);
         check_hip(hipMalloc(reinterpret_cast<void**>(&dev_), dev_cap_), "device alloc");
@@ -228,26 +237,57 @@ void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, dou
         q.m = t.m;
         q.n = t.n;
         q.lq = t.lq;
-        const int64_t mx = t.lq ? t.n : t.m, nx = t.lq ? t.m : t.n, k = std::min(mx, nx);
-        cur += mx * nx + mx * k + k;
+        cur += qr_doubles(t.m, t.n);
         const double M = double(std::max(t.m, t.n)), N = double(std::min(t.m, t.n));
-        pend_units_[QR] += 4.0 * M * N * N - 4.0 * N * N * N / 3.0;
+        const double fl = 4.0 * M * N * N - 4.0 * N * N * N / 3.0;
+        pend_units_[QR] += fl;
         pqr_.push_back(q);
+        pqr_cost_.push_back(fl);
     }
 }
 
-void Batch::flush_qr() {
-    if (pqr_.empty()) return;
-    Cmd c;
-    c.kind = QR;
-    c.tasks = put(pqr_);
-    c.prefix = c.wg = 0;
-    c.count = c.nwg = int64_t(pqr_.size());
-    c.units = pend_units_[QR];
-    pend_units_[QR] = 0;
-    cmds_.push_back(c);
-    pqr_.clear();
+// Split a decomposition launch into an LDS-resident part (working set fits
+// the per-workgroup LDS budget) and a global-scratch part; within each,
+// largest blocks first so the long ones start early.
+template<typename D>
+void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& cost, int64_t lds_max,
+                         int64_t (*doubles)(int64_t, int64_t)) {
+    if (tasks.empty()) return;
+    std::vector<size_t> order(tasks.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    // part 0: global scratch, parts 1..3: LDS classes (largest first)
+    std::vector<D> parts[4];
+    double units[4] = {0, 0, 0, 0};
+    int64_t need[4] = {0, 0, 0, 0};
+    for (size_t i : order) {
+        const int64_t bytes = doubles(tasks[i].m, tasks[i].n) * 8;
+        int p = 0;
+        if (bytes <= lds_max) {
+            p = bytes <= LDS_CLASS[0] ? 3 : (bytes <= LDS_CLASS[1] ? 2 : 1);
+        }
+        parts[p].push_back(tasks[i]);
+        units[p] += cost[i];
+        need[p] = std::max(need[p], bytes);
+    }
+    for (int part = 0; part < 4; part++) {
+        auto& v = parts[part];
+        if (v.empty()) continue;
+        Cmd c;
+        c.kind = kind;
+        c.tasks = put(v);
+        c.prefix = c.wg = 0;
+        c.count = c.nwg = int64_t(v.size());
+        c.units = units[part];
+        c.lds = part == 0 ? 0 : (need[part] + 255) / 256 * 256;
+        cmds_.push_back(c);
+    }
+    pend_units_[kind] = 0;
+    tasks.clear();
+    cost.clear();
 }
+
+void Batch::flush_qr() { flush_decomp(QR, pqr_, pqr_cost_, QR_LDS_MAX, qr_doubles); }
 
 void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch) {
     double* cur = scratch;
@@ -261,24 +301,15 @@ void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, d
         s.m = t.m;
         s.n = t.n;
         const int64_t mx = std::max(t.m, t.n), kx = std::min(t.m, t.n);
-        cur += mx * kx + kx * kx + kx;
-        pend_units_[SVD] += 14.0 * double(mx) * double(kx) * double(kx) + 8.0 * double(kx) * double(kx) * double(kx);
+        cur += svd_doubles(t.m, t.n);
+        const double fl = 14.0 * double(mx) * double(kx) * double(kx) + 8.0 * double(kx) * double(kx) * double(kx);
+        pend_units_[SVD] += fl;
         psvd_.push_back(s);
+        psvd_cost_.push_back(fl);
     }
 }
 
-void Batch::flush_svd() {
-    if (psvd_.empty()) return;
-    Cmd c;
-    c.kind = SVD;
-    c.tasks = put(psvd_);
-    c.prefix = c.wg = 0;
-    c.count = c.nwg = int64_t(psvd_.size());
-    c.units = pend_units_[SVD];
-    pend_units_[SVD] = 0;
-    cmds_.push_back(c);
-    psvd_.clear();
-}
+void Batch::flush_svd() { flush_decomp(SVD, psvd_, psvd_cost_, SVD_LDS_MAX, svd_doubles); }
 
 void Batch::run(hipStream_t st) {
     flush_diag();
@@ -288,7 +319,7 @@ void Batch::run(hipStream_t st) {
     flush_qr();
     flush_svd();
     if (cmds_.empty()) return;
-    char* base = Staging::get().upload(blob_, st);
+    char* base = (staging_ ? *staging_ : Staging::get()).upload(blob_, st);
     for (auto& c : cmds_) {
         hipEvent_t ea = nullptr, eb = nullptr;
         if (timer_) {
@@ -317,10 +348,10 @@ void Batch::run(hipStream_t st) {
                             reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, dtype_, st);
                 break;
             case QR:
-                launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, dtype_, st);
+                launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, c.lds, dtype_, st);
                 break;
             case SVD:
-                launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, dtype_, st);
+                launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, c.lds, dtype_, st);
                 break;
         }
         if (timer_) {
@@ -341,6 +372,8 @@ void Batch::reset() {
     ptile_.clear();
     pqr_.clear();
     psvd_.clear();
+    pqr_cost_.clear();
+    psvd_cost_.clear();
     perm_elems_ = 0;
     for (auto& u : pend_units_) u = 0;
 }

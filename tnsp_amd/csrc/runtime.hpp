@@ -16,10 +16,16 @@ namespace tns {
 
 void check_hip(hipError_t e, const char* what);
 
-// Device/pinned staging for task arrays.  Not thread-safe; one user at a time.
+// Device/pinned staging for task arrays.  Not thread-safe: one user (one
+// stream) at a time; the per-op API shares the process-wide instance, each
+// engine lane owns its own.
 class Staging {
    public:
     static Staging& get();
+    Staging() = default;
+    Staging(const Staging&) = delete;
+    Staging& operator=(const Staging&) = delete;
+    ~Staging();
     // copies blob to device memory on stream st and returns the device base
     char* upload(const std::vector<char>& blob, hipStream_t st);
 
@@ -85,6 +91,7 @@ class Batch {
     int64_t perm_elements() const { return perm_elems_; }
     void reset();
     void set_timer(KernelTimer* t) { timer_ = t; }
+    void set_staging(Staging* s) { staging_ = s; }
 
    private:
     enum Kind { PERM, FILL, GEMM, QR, SVD, DIAG };
@@ -92,9 +99,14 @@ class Batch {
         Kind kind;
         size_t tasks, prefix, wg;  // blob offsets
         int64_t count, nwg;
-        double units;  // algorithmic flops or bytes of this launch
+        double units = 0;  // algorithmic flops or bytes of this launch
+        int64_t lds = 0;   // dynamic LDS bytes (decompositions)
     };
+    template<typename D>
+    void flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& cost, int64_t lds_max,
+                      int64_t (*doubles)(int64_t, int64_t));
     KernelTimer* timer_ = nullptr;
+    Staging* staging_ = nullptr;
     double pend_units_[6] = {0, 0, 0, 0, 0, 0};
     template<typename T>
     size_t put(const std::vector<T>& v);
@@ -109,6 +121,7 @@ class Batch {
     std::vector<TileDev> ptile_;
     std::vector<QrDev> pqr_;
     std::vector<SvdDev> psvd_;
+    std::vector<double> pqr_cost_, psvd_cost_;
     int64_t perm_elems_ = 0;
     double* scratch_cursor_ = nullptr;
 };

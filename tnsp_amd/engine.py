@@ -22,7 +22,7 @@ STAT_NAMES = ["plan_s", "exec_s", "total_s", "gemm_flops", "qr_flops", "svd_flop
 
 
 class AmplitudeEngine:
-    def __init__(self, lattice, threads=None, chunk=1024):
+    def __init__(self, lattice, threads=None, chunk=1024, lanes=1):
         spec = lattice.spec
         self.lattice = lattice  # keeps the site buffers alive
         self.spec = spec
@@ -39,6 +39,11 @@ class AmplitudeEngine:
                                                 int(threads), ctypes.byref(h)))
         self.handle = h
         self.set_chunk(chunk)
+        self.set_lanes(lanes)
+
+    def set_lanes(self, lanes):
+        """Number of concurrent lockstep lanes (own host thread + stream)."""
+        _lib.check(_lib.lib().tns_engine_set_lanes(self.handle, int(lanes)))
 
     def set_chunk(self, chunk):
         _lib.check(_lib.lib().tns_engine_set_chunk(self.handle, int(chunk)))
