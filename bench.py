@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--chunk", type=int, default=0, help="configurations per lockstep pass (0 = batch)")
+    ap.add_argument("--chunk", type=int, default=256, help="configurations per lockstep pass of one lane")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=4, help="concurrent lockstep lanes (host thread + HIP stream each)")
     ap.add_argument("--workload", default="C4", choices=["C2", "C3", "C4"])
@@ -121,8 +121,7 @@ def main():
     storages = peps.random_storages(spec)
     lat = peps.Lattice(spec, storages)
     threads = args.threads or int(os.environ.get("TNS_THREADS", min(16, os.cpu_count() or 1)))
-    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk or -(-args.batch // args.lanes),
-                          lanes=args.lanes)
+    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk, lanes=args.lanes)
     B = args.batch
     total_steps = args.warmup + args.steps
     particles = (24, 40) if spec.kind == "FermiZ2" else None

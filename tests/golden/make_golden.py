@@ -12,6 +12,8 @@ outputs are TAT's own results on those inputs:
 * ``c2_amplitudes.json`` : psi(s) of the 4x4 D=4 Dc=16 FermiU1 PEPS and of a
   3x3 FermiZ2 PEPS for a set of configurations, computed by TAT in
   tetragono's contraction order.
+* ``peps8x8_amplitudes.json`` : the same for the 8x8 FermiZ2 D=6 Dc=36 and
+  the 8x8 FermiU1 D=8 Dc=64 PEPS (a few configurations each).
 
     python tests/golden/make_golden.py
 """
@@ -90,15 +92,11 @@ def storage_digest(storages):
     return h.hexdigest()
 
 
-def make_amplitudes():
+def make_amplitudes(out, specs):
     from tnsp_amd import peps as ppeps  # host-only parts: layout, seeded storages, configurations
 
     cases = {}
-    for name, spec, nconf, particles in (
-        ("C2_4x4_D4_Dc16", ppeps.C2, 24, None),
-        ("Z2_3x3_D4_Dc8", ppeps.LatticeSpec("FermiZ2", 3, 3, (2, 2), 4, 8, seed=21), 12, (2, 6)),
-        ("U1_3x3_D4_exact", ppeps.LatticeSpec("FermiU1", 3, 3, (2, 1, 1), 4, 1000, seed=22), 8, None),
-    ):
+    for name, spec, nconf, particles in specs(ppeps):
         storages = ppeps.random_storages(spec)
         configs = ppeps.sample_configurations(spec, nconf, seed=spec.seed + 100, particles=particles)
         sd = {"kind": spec.kind, "L1": spec.L1, "L2": spec.L2, "N": spec.N, "segments": list(spec.segments)}
@@ -117,13 +115,30 @@ def make_amplitudes():
             psis.append({"value": float(t.data[0]) if t.size else 0.0, "names": t.names})
         cases[name] = {"spec": {**sd, "cut": spec.cut, "seed": spec.seed}, "storage_sha256": storage_digest(storages),
                        "configs": configs.astype(int).tolist(), "psi": psis}
-    with open(os.path.join(HERE, "c2_amplitudes.json"), "w") as f:
+    with open(os.path.join(HERE, out), "w") as f:
         json.dump(cases, f)
+
+
+def small_specs(ppeps):
+    return (
+        ("C2_4x4_D4_Dc16", ppeps.C2, 24, None),
+        ("Z2_3x3_D4_Dc8", ppeps.LatticeSpec("FermiZ2", 3, 3, (2, 2), 4, 8, seed=21), 12, (2, 6)),
+        ("U1_3x3_D4_exact", ppeps.LatticeSpec("FermiU1", 3, 3, (2, 1, 1), 4, 1000, seed=22), 8, None),
+    )
+
+
+def large_specs(ppeps):
+    """The two 8x8 workloads at their full bond and cut dimensions."""
+    return (
+        ("C3_8x8_Z2_D6_Dc36", ppeps.C3, 6, (24, 40)),
+        ("C4_8x8_U1_D8_Dc64", ppeps.C4, 6, None),
+    )
 
 
 if __name__ == "__main__":
     if not rr.available():
         rr.build()
     make_c1()
-    make_amplitudes()
+    make_amplitudes("c2_amplitudes.json", small_specs)
+    make_amplitudes("peps8x8_amplitudes.json", large_specs)
     print("golden fixtures written")

@@ -12,9 +12,10 @@ import itertools
 import numpy as np
 import pytest
 
-from helpers import assert_structure, c1_pair, digest, load_golden, storage, to_gpu
+from helpers import assert_structure, c1_pair, digest, load_golden, load_golden_8x8, storage, to_gpu
 
 OPS, ARRAYS, AMPS = load_golden()
+AMPS8 = load_golden_8x8()
 KINDS = ["FermiU1", "BoseU1", "FermiZ2"]
 
 
@@ -85,7 +86,7 @@ def test_c1_contract_svd_qr(kind, backend):
 def lattice_case(name):
     from tnsp_amd import peps as ppeps
 
-    case = AMPS[name]
+    case = AMPS[name] if name in AMPS else AMPS8[name]
     sd = case["spec"]
     spec = ppeps.LatticeSpec(sd["kind"], sd["L1"], sd["L2"], tuple(sd["segments"]), sd["N"], sd["cut"], sd["seed"])
     storages = ppeps.random_storages(spec)
@@ -96,21 +97,21 @@ def lattice_case(name):
     return spec, storages, np.array(case["configs"]), case["psi"]
 
 
-@pytest.mark.parametrize("name", list(AMPS))
+@pytest.mark.parametrize("name", list(AMPS) + list(AMPS8))
 def test_amplitude_oracle_vs_tat(name):
     from oracle import peps
 
     spec, storages, configs, psis = lattice_case(name)
     sd = {"kind": spec.kind, "L1": spec.L1, "L2": spec.L2, "N": spec.N, "segments": list(spec.segments)}
     sites = peps.build_sites(sd, storages)
-    for cfg, ref in list(zip(configs, psis))[:8]:
+    for cfg, ref in list(zip(configs, psis))[:8 if name in AMPS else 1]:
         v, t = peps.psi(sites, cfg, spec.cut)
         assert t.names == ref["names"]
         assert abs(v - ref["value"]) <= 1e-10 * abs(ref["value"]) + 1e-300
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", list(AMPS))
+@pytest.mark.parametrize("name", list(AMPS) + list(AMPS8))
 def test_amplitude_gpu_plain_and_engine_vs_tat(name):
     from tnsp_amd import peps as ppeps
     from tnsp_amd.engine import AmplitudeEngine
@@ -118,7 +119,7 @@ def test_amplitude_gpu_plain_and_engine_vs_tat(name):
     spec, storages, configs, psis = lattice_case(name)
     lat = ppeps.Lattice(spec, storages)
     ref = np.array([p["value"] for p in psis])
-    for c in range(min(6, len(configs))):
+    for c in range(min(6 if name in AMPS else 1, len(configs))):
         v, t = ppeps.amplitude_plain(lat, configs[c])
         assert t.names == psis[c]["names"]
         assert abs(v - ref[c]) <= 1e-10 * abs(ref[c]) + 1e-300
