@@ -104,6 +104,12 @@ int tns_engine_set_chunk(tns_engine* e, int64_t chunk);
  * host thread and HIP stream; configurations are split over them so one
  * lane's host planning and decomposition tails overlap the others' kernels */
 int tns_engine_set_lanes(tns_engine* e, int lanes);
+/* precision of the contraction GEMMs of a float64 engine: 0 fp64, 1 fp32,
+ * 2 split-bf16 x3, 3 split-bf16 x6 (fp32 accumulation); operands are
+ * converted on the way into the GEMM and the product back to float64, QR, SVD
+ * and all stored tensors stay float64.  No reference counterpart (TAT
+ * computes in the tensor's own scalar type). */
+int tns_engine_set_precision(tns_engine* e, int precision);
 int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
                         double relative_cut, double* psi, void* stream);
 /* values: plan_s exec_s total_s gemm_flops qr_flops svd_flops perm_elems
@@ -116,6 +122,34 @@ int tns_engine_reset_stats(tns_engine* e);
 int tns_engine_set_profile(tns_engine* e, int on);
 int tns_engine_kernel_stats(const tns_engine* e, double* values, int n);
 int tns_engine_free(tns_engine* e);
+
+/* hop-proposal network for Metropolis sampling (no reference counterpart:
+ * the reference proposes hops bond by bond, sampling.py:117-154).  NCHW
+ * float32 activations, bf16 weights (uint16 bit patterns) [cout][cin][3][3],
+ * float32 accumulation, zero padding.  relu_y: the layer's ReLU output whose
+ * mask gates dy (NULL = no ReLU).  Hops are d * H * W + site with d = 0 up,
+ * 1 down, 2 left, 3 right; a hop is valid when the site is filled and its
+ * neighbour exists and is empty. */
+int tns_conv3x3_forward(const float* x, const uint16_t* w, const float* bias, float* y, int batch, int cin, int cout,
+                        int height, int width, int relu, void* stream);
+int tns_conv3x3_backward_data(const float* dy, const float* relu_y, const uint16_t* w, float* dx, int batch, int cin,
+                              int cout, int height, int width, void* stream);
+int tns_conv3x3_backward_weight_workspace(int batch, int cin, int cout, int64_t* bytes);
+int tns_conv3x3_backward_weight(const float* dy, const float* relu_y, const float* x, float* dw, float* dbias,
+                                int batch, int cin, int cout, int height, int width, void* workspace, void* stream);
+/* masked softmax over valid hops: draw one per configuration (inverse CDF
+ * of uniforms in [0, 1)) or score the given ones; log_q = log q(hop) */
+int tns_hop_sample(const float* logits, const int8_t* occupations, const float* uniforms, int32_t* hops,
+                   float* log_q, int batch, int height, int width, void* stream);
+int tns_hop_log_prob(const float* logits, const int8_t* occupations, int32_t* hops, float* log_q, int batch,
+                     int height, int width, void* stream);
+/* loss_b = (logit[b, hop_b] - target_b)^2, dlogits = d mean(loss) / d logits */
+int tns_hop_regression(const float* logits, const int32_t* hops, const float* targets, float* dlogits, float* loss,
+                       int batch, int n, void* stream);
+int tns_adam_step(float* w, float* m, float* v, const float* grad, uint16_t* w_bf16, int n, float lr, float beta1,
+                  float beta2, float eps, int step, void* stream);
+int tns_to_bf16(const float* w, uint16_t* w_bf16, int n, void* stream);
+int tns_occupations_to_input(const int8_t* occupations, float* x, int64_t n, void* stream);
 
 /* planner introspection used by the CPU test-suite: element counts of the
  * plan's logical buffers (-1 = caller-owned input), the sector table of an

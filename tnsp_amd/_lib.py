@@ -53,12 +53,24 @@ _SIGNATURES = {
                           ctypes.c_int, c_vpp],
     "tns_engine_set_chunk": [c_vp, ctypes.c_int64],
     "tns_engine_set_lanes": [c_vp, ctypes.c_int],
+    "tns_engine_set_precision": [c_vp, ctypes.c_int],
     "tns_engine_evaluate": [c_vp, ctypes.c_int64, c_i32p, ctypes.c_int64, ctypes.c_double, c_dblp, c_vp],
     "tns_engine_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_reset_stats": [c_vp],
     "tns_engine_set_profile": [c_vp, ctypes.c_int],
     "tns_engine_kernel_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_free": [c_vp],
+    "tns_conv3x3_forward": [c_vp, c_vp, c_vp, c_vp] + [ctypes.c_int] * 6 + [c_vp],
+    "tns_conv3x3_backward_data": [c_vp, c_vp, c_vp, c_vp] + [ctypes.c_int] * 5 + [c_vp],
+    "tns_conv3x3_backward_weight_workspace": [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_i64p],
+    "tns_conv3x3_backward_weight": [c_vp] * 5 + [ctypes.c_int] * 5 + [c_vp, c_vp],
+    "tns_hop_sample": [c_vp] * 5 + [ctypes.c_int] * 3 + [c_vp],
+    "tns_hop_log_prob": [c_vp] * 4 + [ctypes.c_int] * 3 + [c_vp],
+    "tns_hop_regression": [c_vp] * 5 + [ctypes.c_int] * 2 + [c_vp],
+    "tns_adam_step": [c_vp] * 5 + [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_int, c_vp],
+    "tns_to_bf16": [c_vp, c_vp, ctypes.c_int, c_vp],
+    "tns_occupations_to_input": [c_vp, c_vp, ctypes.c_int64, c_vp],
     "tns_plan_stage_count": [c_vp, ctypes.c_int, c_i64p],
     "tns_plan_stage_export": [c_vp, ctypes.c_int, c_i64p],
 }

@@ -174,11 +174,15 @@ void Engine::contract(std::vector<Slot*>& a, std::vector<Slot*>& b,
     arenas_[A_TEMP].reset();
     keep_[A_TEMP] = 0;
     keep_[oa] = arenas_[oa].used;
+    // reduced precision: the merged operands and the product are float32
+    // (converted by the permutes), everything stored stays float64
+    const bool lowp = gemm_prec_ != 0;
+    const int64_t eg = lowp ? 4 : es_;
     std::vector<int64_t> ta(B), tb(B), tp(B), to(B);
     for (int64_t i = 0; i < B; i++) {
-        ta[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].ma.size * es_)));
-        tb[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].mb.size * es_)));
-        tp[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].prod.size * es_)));
+        ta[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].ma.size * eg)));
+        tb[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].mb.size * eg)));
+        tp[i] = int64_t(arenas_[A_TEMP].alloc(size_t(plans[i].prod.size * eg)));
         to[i] = int64_t(arenas_[oa].alloc(size_t(plans[i].result.size * es_)));
         stats_.gemm_flops += double(plans[i].flops);
     }
@@ -188,25 +192,45 @@ void Engine::contract(std::vector<Slot*>& a, std::vector<Slot*>& b,
     char* T = arenas_[A_TEMP].base;
     for (int64_t i = 0; i < B; i++) {
         void* bufs[6] = {ptr(*a[i]), ptr(*b[i]), T + ta[i], T + tb[i], T + tp[i], arenas_[oa].base + to[i]};
-        bt.perm(plans[i].perm_a, bufs, es_);
-        bt.perm(plans[i].perm_b, bufs, es_);
+        if (lowp) {
+            bt.perm_cvt(plans[i].perm_a, bufs, es_, eg);
+            bt.perm_cvt(plans[i].perm_b, bufs, es_, eg);
+        } else {
+            bt.perm(plans[i].perm_a, bufs, es_);
+            bt.perm(plans[i].perm_b, bufs, es_);
+        }
     }
-    bt.flush_perm();
+    if (lowp)
+        bt.flush_perm_cvt(1);
+    else
+        bt.flush_perm();
     for (int64_t i = 0; i < B; i++) {
         void* bufs[6] = {nullptr, nullptr, T + ta[i], T + tb[i], T + tp[i], nullptr};
-        bt.fill(plans[i].fills, bufs, es_);
+        bt.fill(plans[i].fills, bufs, eg);
     }
-    bt.flush_fill();
+    if (lowp)
+        bt.flush_fill_as(1);
+    else
+        bt.flush_fill();
     for (int64_t i = 0; i < B; i++) {
         void* bufs[6] = {nullptr, nullptr, T + ta[i], T + tb[i], T + tp[i], nullptr};
-        bt.gemm(plans[i].gemms, bufs, es_);
+        bt.gemm(plans[i].gemms, bufs, eg);
     }
-    bt.flush_gemm();
+    if (lowp)
+        bt.flush_gemm_as(gemm_prec_);
+    else
+        bt.flush_gemm();
     for (int64_t i = 0; i < B; i++) {
         void* bufs[6] = {nullptr, nullptr, T + ta[i], T + tb[i], T + tp[i], arenas_[oa].base + to[i]};
-        bt.perm(plans[i].perm_out, bufs, es_);
+        if (lowp)
+            bt.perm_cvt(plans[i].perm_out, bufs, eg, es_);
+        else
+            bt.perm(plans[i].perm_out, bufs, es_);
     }
-    bt.flush_perm();
+    if (lowp)
+        bt.flush_perm_cvt(2);
+    else
+        bt.flush_perm();
     stats_.plan_s += now() - t0;
     stats_.launches += 4;
     run_batch(bt, st);
@@ -311,6 +335,7 @@ void Engine::svd(std::vector<Slot*>& t, const std::vector<int32_t>& free_u, int3
     grow_all(st);
     t0 = now();
     Batch bt(dtype_);
+    bt.set_fast_svd(true);
     char* T = arenas_[A_TEMP].base;
     char* S = arenas_[A_SVALS].base;
     for (int64_t i = 0; i < B; i++) {
@@ -635,6 +660,7 @@ struct tns_engine {
     std::vector<const void*> data;
     std::vector<int32_t> names;
     int64_t chunk = 1024;
+    int precision = 0;
     bool profile = false;
     std::vector<std::unique_ptr<tns::Engine>> lanes;
     std::vector<hipStream_t> streams;  // streams[0] unused (caller's stream)
@@ -648,6 +674,7 @@ struct tns_engine {
         while (int(lanes.size()) < n) {
             lanes.push_back(std::make_unique<tns::Engine>(kind, L1, L2, sites, data, dtype, names, per));
             lanes.back()->set_chunk(chunk);
+            lanes.back()->set_precision(precision);
             lanes.back()->set_profile(profile);
             hipStream_t s = nullptr;
             if (lanes.size() > 1) tns::check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
@@ -683,6 +710,15 @@ int tns_engine_set_chunk(tns_engine* e, int64_t chunk) {
     TNS_TRY({
         e->chunk = chunk;
         for (auto& l : e->lanes) l->set_chunk(chunk);
+    })
+}
+
+int tns_engine_set_precision(tns_engine* e, int precision) {
+    TNS_TRY({
+        if (precision < 0 || precision > 3) throw tns::Error("precision must be 0 (fp64), 1 (fp32), 2 (bf16x3) or 3 (bf16x6)");
+        if (precision != 0 && e->dtype != 0) throw tns::Error("reduced-precision GEMMs need a float64 engine");
+        e->precision = precision;
+        for (auto& l : e->lanes) l->set_precision(precision);
     })
 }
 

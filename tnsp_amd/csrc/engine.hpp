@@ -78,6 +78,10 @@ class Engine {
     void evaluate(int64_t nconf, const int32_t* occupations, int64_t remain_cut, double relative_cut, double* psi,
                   hipStream_t st);
     void set_chunk(int64_t c) { chunk_ = c > 0 ? c : 1; }
+    // contraction GEMM precision with float64 storage: 0 fp64, 1 fp32,
+    // 2 split-bf16 x3, 3 split-bf16 x6 (QR, SVD and every stored tensor stay
+    // float64)
+    void set_precision(int p) { gemm_prec_ = dtype_ == 0 ? p : 0; }
     const EngineStats& stats() const { return stats_; }
     void reset_stats() {
         stats_ = EngineStats();
@@ -115,6 +119,7 @@ class Engine {
     size_t keep_[NARENA] = {0, 0, 0, 0, 0};
     Pool pool_;
     int64_t chunk_ = 1024;
+    int gemm_prec_ = 0;
     int64_t remain_cut_ = -1;
     double relative_cut_ = 0.0;
     EngineStats stats_;

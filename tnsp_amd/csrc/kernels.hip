@@ -98,6 +98,30 @@ struct Mfma<float> {
 
 constexpr int KB = 16;
 
+// split-bf16 products on the bf16 MFMA (16x16x16, fp32 accumulation).  An
+// fp32 value is split into NTERM bf16 terms (round to nearest even, each
+// term taken from the remainder of the previous ones); all term products of
+// order i + j < NTERM are accumulated, smallest first.
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_rne(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+__device__ inline float bf16_value(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+template<int NTERM>
+__device__ inline void bf16_split(float x, s4* terms, int j) {
+#pragma unroll
+    for (int t = 0; t < NTERM; t++) {
+        const unsigned short h = bf16_rne(x);
+        terms[t][j] = static_cast<short>(h);
+        x -= bf16_value(h);
+    }
+}
+
 // C tile 32x32 per workgroup; wave w owns the 16x16 sub-tile (w/2, w%2).
 // MFMA 16x16x4 operand layout: lane l holds A(l%16, l/16) and B(l/16, l%16);
 // accumulator rows as given by Mfma<T>::row.
@@ -158,6 +182,76 @@ __global__ __launch_bounds__(WG) void gemm_kernel(const GemmDev* __restrict__ ta
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int r = row0 + wr + Mfma<T>::row(lane, i);
+        const int c = col0 + wc + lane % 16;
+        if (r < g.m && c < g.n) C[int64_t(r) * g.ldc + c] = alpha * acc[i];
+    }
+}
+
+// Same tiling as gemm_kernel<float>, with each 16-deep k step done as
+// split-bf16 MFMA products.  Lane l holds A(l % 16, 4 (l / 16) + j) and
+// B(4 (l / 16) + j, l % 16), j = 0..3; accumulator rows 4 (l / 16) + i.
+template<int NTERM>
+__global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict__ tasks,
+                                                       const TileDev* __restrict__ tiles) {
+    __shared__ float As[GEMM_TILE][KB + 1];
+    __shared__ float Bs[KB][GEMM_TILE + 1];
+    const TileDev tile = tiles[blockIdx.x];
+    const GemmDev g = tasks[tile.task];
+    const float* A = static_cast<const float*>(g.a);
+    const float* B = static_cast<const float*>(g.b);
+    const int row0 = tile.tm * GEMM_TILE, col0 = tile.tn * GEMM_TILE;
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const int wr = (wave / 2) * 16, wc = (wave % 2) * 16;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.k; k0 += KB) {
+        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+            int i, p;
+            if (g.ta) {
+                i = e % GEMM_TILE;
+                p = e / GEMM_TILE;
+            } else {
+                p = e % KB;
+                i = e / KB;
+            }
+            const int gi = row0 + i, gp = k0 + p;
+            float v = 0.f;
+            if (gi < g.m && gp < g.k) v = g.ta ? A[int64_t(gp) * g.lda + gi] : A[int64_t(gi) * g.lda + gp];
+            As[i][p] = v;
+        }
+        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+            int p, j;
+            if (g.tb) {
+                p = e % KB;
+                j = e / KB;
+            } else {
+                j = e % GEMM_TILE;
+                p = e / GEMM_TILE;
+            }
+            const int gp = k0 + p, gj = col0 + j;
+            float v = 0.f;
+            if (gp < g.k && gj < g.n) v = g.tb ? B[int64_t(gj) * g.ldb + gp] : B[int64_t(gp) * g.ldb + gj];
+            Bs[p][j] = v;
+        }
+        __syncthreads();
+        s4 at[NTERM], bt[NTERM];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            bf16_split<NTERM>(As[wr + lane % 16][4 * (lane / 16) + j], at, j);
+            bf16_split<NTERM>(Bs[4 * (lane / 16) + j][wc + lane % 16], bt, j);
+        }
+#pragma unroll
+        for (int order = NTERM - 1; order >= 0; order--) {
+#pragma unroll
+            for (int i = 0; i <= order; i++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(at[i], bt[order - i], acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float* C = static_cast<float*>(g.c);
+    const float alpha = float(g.alpha);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int r = row0 + wr + 4 * (lane / 16) + i;
         const int c = col0 + wc + lane % 16;
         if (r < g.m && c < g.n) C[int64_t(r) * g.ldc + c] = alpha * acc[i];
     }
@@ -322,6 +416,38 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_glb(const QrDev* __restrict__
 }
 
 // -------------------------------------------------------------------- svd
+// Ranks the singular values (descending, ties by column) and writes S, U
+// and V^T of the block from X (normalised columns), V and sig.
+template<typename T>
+__device__ __forceinline__ void svd_write(const SvdDev& tk, const double* X, const double* V, const double* sig,
+                                          int g, int gl, int G, int NG) {
+    const int m = tk.m, n = tk.n;
+    const bool trans = m < n;
+    const int mx = trans ? n : m, kx = trans ? m : n;
+    T* U = static_cast<T*>(tk.u);
+    T* Vo = static_cast<T*>(tk.v);
+    for (int c = g; c < kx; c += NG) {
+        const double sc = sig[c];
+        int rank = 0;
+        for (int d = 0; d < kx; d++) {
+            const double sd = sig[d];
+            rank += (sd > sc) || (sd == sc && d < c);
+        }
+        if (gl == 0) tk.s[rank] = sc;
+        const double* xc = X + int64_t(c) * mx;
+        const double* vc = V + int64_t(c) * kx;
+        if (!trans) {
+            // M = X diag(sig) V^T with X orthonormal: U (m x k), Vo = V^T
+            for (int r = gl; r < m; r += G) U[int64_t(r) * kx + rank] = T(xc[r]);
+            for (int i = gl; i < n; i += G) Vo[int64_t(rank) * n + i] = T(vc[i]);
+        } else {
+            // M^T = X diag(sig) V^T  =>  M = V diag(sig) X^T
+            for (int i = gl; i < m; i += G) U[int64_t(i) * kx + rank] = T(vc[i]);
+            for (int r = gl; r < n; r += G) Vo[int64_t(rank) * n + r] = T(xc[r]);
+        }
+    }
+}
+
 // One-sided Jacobi (Hestenes) on the kx = min(m, n) columns of X (X = M when
 // m >= n, X = M^T otherwise), round-robin pair ordering; each pair is handled
 // by one group of G lanes.  X, V (kx x kx) and sigma live in LDS when they
@@ -466,28 +592,7 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
         }
     }
     __syncthreads();
-    T* U = static_cast<T*>(tk.u);
-    T* Vo = static_cast<T*>(tk.v);
-    for (int c = g; c < kx; c += NG) {
-        const double sc = sig[c];
-        int rank = 0;
-        for (int d = 0; d < kx; d++) {
-            const double sd = sig[d];
-            rank += (sd > sc) || (sd == sc && d < c);
-        }
-        if (gl == 0) tk.s[rank] = sc;
-        const double* xc = X + int64_t(c) * mx;
-        const double* vc = V + int64_t(c) * kx;
-        if (!trans) {
-            // M = X diag(sig) V^T with X orthonormal: U (m x k), Vo = V^T
-            for (int r = gl; r < m; r += G) U[int64_t(r) * kx + rank] = T(xc[r]);
-            for (int i = gl; i < n; i += G) Vo[int64_t(rank) * n + i] = T(vc[i]);
-        } else {
-            // M^T = X diag(sig) V^T  =>  M = V diag(sig) X^T
-            for (int i = gl; i < m; i += G) U[int64_t(i) * kx + rank] = T(vc[i]);
-            for (int r = gl; r < n; r += G) Vo[int64_t(rank) * n + r] = T(xc[r]);
-        }
-    }
+    svd_write<T>(tk, X, V, sig, g, gl, G, NG);
 }
 
 template<typename T>
@@ -506,6 +611,203 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
     const int mx = tk.m < tk.n ? tk.n : tk.m, kx = tk.m < tk.n ? tk.m : tk.n;
     double* X = tk.work;
     svd_body<T>(tk, X, X + int64_t(mx) * kx, X + int64_t(mx) * kx + int64_t(kx) * kx, &flag);
+}
+
+
+// Register-resident one-sided Jacobi for the large blocks of the batched
+// amplitude engine (rows mx <= 8 * RPT, so columns kx <= 8 * RPT).  32 * RPT
+// threads; pair slot p (4 * RPT slots) is held by 8 lanes of one wave
+// (lane = 8 * row_group + slot % 8),
+// each keeping RPT rows of the slot's two columns in registers.  After every
+// round the columns move along the round-robin ring (slot 0's top fixed,
+// tops shift up, bottoms shift down) through LDS in chunks of rows.
+// V is not accumulated: after convergence it is recovered as
+// V = M^T U diag(1/sigma) (an LDS-tiled product), which is exact up to
+// eps * sigma_max / sigma_j; columns with sigma_j <= 1e-13 * sigma_max get
+// V = 0 (their products U S V^T are below that level).  The engine only
+// consumes U S and V^T of kept values, so no orthonormal completion is done.
+
+__device__ inline double sum8(double v) {
+    v += __shfl_xor(v, 8, WAVE);
+    v += __shfl_xor(v, 16, WAVE);
+    v += __shfl_xor(v, 32, WAVE);
+    return v;
+}
+
+template<typename T, int RPT>
+__global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restrict__ tasks) {
+    // rows exchanged per LDS pass: smaller for the largest tile, which keeps
+    // the kernel within the 168 registers of three waves per SIMD
+    constexpr int SVR_NT = 32 * RPT, SVR_SLOTS = 4 * RPT, SVR_CH = RPT >= 24 ? 4 : 8;
+    extern __shared__ double xbuf[];  // 2 x SVR_CH x SVR_SLOTS x 8 doubles
+    __shared__ int ids[2][SVR_SLOTS];
+    __shared__ double smax;
+    const SvdDev tk = tasks[blockIdx.x];
+    const int m = tk.m, n = tk.n;
+    const bool trans = m < n;
+    const int mx = trans ? n : m, kx = trans ? m : n;
+    const T* in = static_cast<const T*>(tk.in);
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    if (mx > 8 * RPT || kx > 2 * SVR_SLOTS) {
+        // the host routes only fitting blocks here; flag anything else
+        for (int c = tid; c < kx; c += SVR_NT) tk.s[c] = __builtin_nan("");
+        return;
+    }
+    const int gl = lane / 8, p = w * 8 + lane % 8;
+    const int h = (kx + 1) / 2;
+    const bool act = p < h;
+    int ca = p, cb = p + h;
+    double xa[RPT], xb[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const int r = gl + 8 * i;
+        xa[i] = (act && r < mx && ca < kx) ? double(trans ? in[int64_t(ca) * n + r] : in[int64_t(r) * n + ca]) : 0.0;
+        xb[i] = (act && r < mx && cb < kx) ? double(trans ? in[int64_t(cb) * n + r] : in[int64_t(r) * n + cb]) : 0.0;
+    }
+    const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
+    for (int sweep = 0; sweep < 60 && kx > 1; sweep++) {
+        int rot = 0;
+        for (int round = 0; round < 2 * h - 1; round++) {
+            if (act) {
+                double saa = 0, sbb = 0, sab = 0;
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    saa += xa[i] * xa[i];
+                    sbb += xb[i] * xb[i];
+                    sab += xa[i] * xb[i];
+                }
+                saa = sum8(saa);
+                sbb = sum8(sbb);
+                sab = sum8(sab);
+                if (ca < kx && cb < kx && sab != 0.0 && fabs(sab) > tol * sqrt(saa * sbb)) {
+                    const double zeta = (sbb - saa) / (2.0 * sab);
+                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        const double u = xa[i], v = xb[i];
+                        xa[i] = cs * u - sn * v;
+                        xb[i] = sn * u + cs * v;
+                    }
+                    rot = 1;
+                }
+            }
+            if (h < 2) continue;
+            // ring step: top[p] <- top[p-1] (top[1] <- bot[0], top[0] stays),
+            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1])
+#pragma unroll
+            for (int c0 = 0; c0 < RPT; c0 += SVR_CH) {
+                if (act) {
+#pragma unroll
+                    for (int i = 0; i < SVR_CH; i++) {
+                        xbuf[((0 * SVR_CH + i) * SVR_SLOTS + p) * 8 + gl] = p == 0 ? xb[c0 + i] : xa[c0 + i];
+                        xbuf[((1 * SVR_CH + i) * SVR_SLOTS + p) * 8 + gl] = xb[c0 + i];
+                    }
+                    if (c0 == 0 && gl == 0) {
+                        ids[0][p] = p == 0 ? cb : ca;
+                        ids[1][p] = cb;
+                    }
+                }
+                __syncthreads();
+                if (act) {
+#pragma unroll
+                    for (int i = 0; i < SVR_CH; i++) {
+                        const double told = xa[c0 + i];
+                        if (p >= 1) xa[c0 + i] = xbuf[((0 * SVR_CH + i) * SVR_SLOTS + p - 1) * 8 + gl];
+                        xb[c0 + i] = p <= h - 2 ? xbuf[((1 * SVR_CH + i) * SVR_SLOTS + p + 1) * 8 + gl] : told;
+                    }
+                    if (c0 == 0) {
+                        const int nca = p >= 1 ? ids[0][p - 1] : ca;
+                        const int ncb = p <= h - 2 ? ids[1][p + 1] : ca;
+                        ca = nca;
+                        cb = ncb;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (!__syncthreads_or(rot)) break;
+    }
+    // sigma, normalised U columns into scratch
+    double* X = tk.work;
+    double* V = X + int64_t(mx) * kx;
+    double* sig = V + int64_t(kx) * kx;
+    if (tid == 0) smax = 0.0;
+    __syncthreads();
+    if (act) {
+        double sa = 0, sb = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            sa += xa[i] * xa[i];
+            sb += xb[i] * xb[i];
+        }
+        sa = sqrt(sum8(sa));
+        sb = sqrt(sum8(sb));
+        const double ia = sa > 0 ? 1.0 / sa : 0.0, ib = sb > 0 ? 1.0 / sb : 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const int r = gl + 8 * i;
+            if (r < mx) {
+                if (ca < kx) X[int64_t(ca) * mx + r] = xa[i] * ia;
+                if (cb < kx) X[int64_t(cb) * mx + r] = xb[i] * ib;
+            }
+        }
+        if (gl == 0) {
+            if (ca < kx) sig[ca] = sa;
+            if (cb < kx) sig[cb] = sb;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double mxs = 0;
+        for (int c = 0; c < kx; c++) mxs = fmax(mxs, sig[c]);
+        smax = mxs;
+    }
+    __syncthreads();
+    // V(c, j) = sum_r M0(r, c) U(r, j) / sig_j, with M0 = M (mx = m) or M^T
+    constexpr int TB = 64, KB = 32;
+    double* As = xbuf;            // KB x TB: M0 rows r0.., columns c0..
+    double* Bs = xbuf + KB * TB;  // KB x TB: U rows r0.., columns j0..
+    const double vthr = 1e-13 * smax;
+    for (int c0 = 0; c0 < kx; c0 += TB) {
+        for (int j0 = 0; j0 < kx; j0 += TB) {
+            // thread: column c0 + ci, rows j0 + jg + q * JS of the V tile
+            constexpr int JS = SVR_NT / TB, JN = (TB + JS - 1) / JS;
+            const int ci = tid % TB, jg = tid / TB;
+            double acc[JN];
+#pragma unroll
+            for (int q = 0; q < JN; q++) acc[q] = 0.0;
+            for (int r0 = 0; r0 < mx; r0 += KB) {
+                for (int e = tid; e < KB * TB; e += SVR_NT) {
+                    const int rr = e / TB, cc = e % TB, r = r0 + rr;
+                    const int c = c0 + cc, j = j0 + cc;
+                    As[e] = (r < mx && c < kx) ? double(trans ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]) : 0.0;
+                    Bs[e] = (r < mx && j < kx) ? X[int64_t(j) * mx + r] : 0.0;
+                }
+                __syncthreads();
+#pragma unroll 8
+                for (int rr = 0; rr < KB; rr++) {
+                    const double a = As[rr * TB + ci];
+#pragma unroll
+                    for (int q = 0; q < JN; q++)
+                        if (jg + q * JS < TB) acc[q] += a * Bs[rr * TB + jg + q * JS];
+                }
+                __syncthreads();
+            }
+            const int c = c0 + ci;
+            if (c < kx) {
+#pragma unroll
+                for (int q = 0; q < JN; q++) {
+                    const int j = j0 + jg + q * JS;
+                    if (jg + q * JS < TB && j < kx) V[int64_t(j) * kx + c] = sig[j] > vthr ? acc[q] / sig[j] : 0.0;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int G = 8, NG = SVR_NT / G;
+    svd_write<T>(tk, X, V, sig, tid / G, tid % G, G, NG);
 }
 
 inline void check(hipError_t e, const char* what) {
@@ -548,12 +850,26 @@ void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_
     check(hipGetLastError(), "fill launch");
 }
 
+void launch_permute_to_f64(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg,
+                           hipStream_t stream) {
+    if (num_wg <= 0) return;
+    hipLaunchKernelGGL((permute_kernel<float, double>), dim3(unsigned(num_wg)), dim3(WG), 0, stream, tasks, prefix,
+                       wg_first);
+    check(hipGetLastError(), "permute launch");
+}
+
 void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream) {
     if (num_tiles <= 0) return;
     if (dtype == 0)
         hipLaunchKernelGGL(gemm_kernel<double>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
-    else
+    else if (dtype == 1)
         hipLaunchKernelGGL(gemm_kernel<float>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
+    else if (dtype == 2)
+        hipLaunchKernelGGL(gemm_kernel_bf16<2>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
+    else if (dtype == 3)
+        hipLaunchKernelGGL(gemm_kernel_bf16<3>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
+    else
+        throw std::runtime_error("gemm: unsupported precision");
     check(hipGetLastError(), "gemm launch");
 }
 
@@ -587,9 +903,32 @@ void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hi
     check(hipGetLastError(), "qr launch");
 }
 
-void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream) {
+namespace {
+template<typename T, int RPT>
+void launch_svd_reg(const SvdDev* tasks, int64_t num, hipStream_t stream) {
+    // exchange buffer (2 x 8 rows x slots x 8 lanes), at least the 32 KiB of
+    // the recovery tiles
+    const size_t ch = RPT >= 24 ? 4 : 8;
+    const size_t lds = std::max<size_t>(2 * ch * 4 * RPT * 8, 2 * 32 * 64) * sizeof(double);
+    allow_lds(svd_kernel_reg<T, RPT>, int64_t(lds));
+    hipLaunchKernelGGL((svd_kernel_reg<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
+}
+}  // namespace
+
+void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (lds_bytes > 0) {
+    if (rpt > 0) {
+        if (rpt != 8 && rpt != 16 && rpt != 24) throw std::runtime_error("svd: unsupported register tile");
+        if (dtype == 0) {
+            if (rpt == 8) launch_svd_reg<double, 8>(tasks, num, stream);
+            if (rpt == 16) launch_svd_reg<double, 16>(tasks, num, stream);
+            if (rpt == 24) launch_svd_reg<double, 24>(tasks, num, stream);
+        } else {
+            if (rpt == 8) launch_svd_reg<float, 8>(tasks, num, stream);
+            if (rpt == 16) launch_svd_reg<float, 16>(tasks, num, stream);
+            if (rpt == 24) launch_svd_reg<float, 24>(tasks, num, stream);
+        }
+    } else if (lds_bytes > 0) {
         if (dtype == 0) {
             allow_lds(svd_kernel_lds<double>, lds_bytes);
             hipLaunchKernelGGL(svd_kernel_lds<double>, dim3(unsigned(num)), dim3(SVD_WG), size_t(lds_bytes), stream,

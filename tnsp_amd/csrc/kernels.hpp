@@ -66,6 +66,11 @@ void launch_permute_from_f64(const PermDev* tasks, const int64_t* prefix, const 
 void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg, int dtype,
                  hipStream_t stream);
 constexpr int GEMM_TILE = 32;
+void launch_permute_to_f64(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg,
+                           hipStream_t stream);
+// dtype: 0 float64, 1 float32, 2 float32 operands as split-bf16 x3 (hi*hi +
+// hi*lo + lo*hi), 3 split-bf16 x6 (three-term split, all products of order
+// <= 2); float32 accumulation for 1..3
 void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream);
 // QR / SVD: one workgroup per block.  lds_bytes > 0 selects the LDS-resident
 // kernel with that much dynamic LDS (every task of the launch must fit);
@@ -87,6 +92,10 @@ inline int64_t svd_doubles(int64_t m, int64_t n) {
     return mx * kx + kx * kx + kx;
 }
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream);
-void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream);
+// rpt > 0: register-resident kernel for blocks with at most 8 * rpt rows
+// (rpt in {8, 16, 24}), V recovered from U; otherwise LDS (lds_bytes > 0)
+// or global scratch.
+void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, int dtype, hipStream_t stream);
+constexpr int SVD_REG_MAX_ROWS = 192;
 
 }  // namespace tns

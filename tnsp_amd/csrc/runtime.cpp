@@ -177,6 +177,33 @@ void Batch::flush_perm() {
     flush_flat(PERM, pperm_, none);
 }
 
+void Batch::perm_cvt(const std::vector<PermTask>& ts, void* const* bufs, int64_t es_src, int64_t es_dst) {
+    for (auto& t : ts)
+        if (t.n > 0) {
+            pperm_.push_back(bind_perm(t, bufs, es_src, es_dst));
+            perm_elems_ += t.n;
+            pend_units_[PERM] += double(t.n * (es_src + es_dst));
+        }
+}
+
+void Batch::flush_perm_cvt(int conv) {
+    const size_t before = cmds_.size();
+    flush_perm();
+    if (cmds_.size() > before) cmds_.back().conv = conv;
+}
+
+void Batch::flush_fill_as(int dt) {
+    const size_t before = cmds_.size();
+    flush_fill();
+    if (cmds_.size() > before) cmds_.back().dt = dt;
+}
+
+void Batch::flush_gemm_as(int prec) {
+    const size_t before = cmds_.size();
+    flush_gemm();
+    if (cmds_.size() > before) cmds_.back().dt = prec;
+}
+
 void Batch::flush_fill() {
     std::vector<PermDev> none;
     flush_flat(FILL, none, pfill_);
@@ -256,21 +283,26 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     std::vector<size_t> order(tasks.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    // part 0: global scratch, parts 1..3: LDS classes (largest first)
-    std::vector<D> parts[4];
-    double units[4] = {0, 0, 0, 0};
-    int64_t need[4] = {0, 0, 0, 0};
+    // part 0: global scratch, parts 1..3: LDS classes (largest first),
+    // parts 4..6: register-resident SVD with 24 / 16 / 8 rows per lane
+    std::vector<D> parts[7];
+    double units[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t need[7] = {0, 0, 0, 0, 0, 0, 0};
+    static const int part_rpt[7] = {0, 0, 0, 0, 24, 16, 8};
     for (size_t i : order) {
         const int64_t bytes = doubles(tasks[i].m, tasks[i].n) * 8;
+        const int64_t rows = std::max<int64_t>(tasks[i].m, tasks[i].n);
         int p = 0;
-        if (bytes <= lds_max) {
+        if (kind == SVD && fast_svd_ && bytes > LDS_CLASS[1] && rows <= SVD_REG_MAX_ROWS) {
+            p = rows <= 64 ? 6 : (rows <= 128 ? 5 : 4);
+        } else if (bytes <= lds_max) {
             p = bytes <= LDS_CLASS[0] ? 3 : (bytes <= LDS_CLASS[1] ? 2 : 1);
         }
         parts[p].push_back(tasks[i]);
         units[p] += cost[i];
         need[p] = std::max(need[p], bytes);
     }
-    for (int part = 0; part < 4; part++) {
+    for (int part = 0; part < 7; part++) {
         auto& v = parts[part];
         if (v.empty()) continue;
         Cmd c;
@@ -279,7 +311,8 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
         c.prefix = c.wg = 0;
         c.count = c.nwg = int64_t(v.size());
         c.units = units[part];
-        c.lds = part == 0 ? 0 : (need[part] + 255) / 256 * 256;
+        c.lds = (part == 0 || part >= 4) ? 0 : (need[part] + 255) / 256 * 256;
+        c.rpt = part_rpt[part];
         cmds_.push_back(c);
     }
     pend_units_[kind] = 0;
@@ -329,9 +362,18 @@ void Batch::run(hipStream_t st) {
         }
         switch (c.kind) {
             case PERM:
-                launch_permute(reinterpret_cast<const PermDev*>(base + c.tasks),
-                               reinterpret_cast<const int64_t*>(base + c.prefix),
-                               reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
+                if (c.conv == 1)
+                    launch_permute_from_f64(reinterpret_cast<const PermDev*>(base + c.tasks),
+                                            reinterpret_cast<const int64_t*>(base + c.prefix),
+                                            reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, 1, st);
+                else if (c.conv == 2)
+                    launch_permute_to_f64(reinterpret_cast<const PermDev*>(base + c.tasks),
+                                          reinterpret_cast<const int64_t*>(base + c.prefix),
+                                          reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, st);
+                else
+                    launch_permute(reinterpret_cast<const PermDev*>(base + c.tasks),
+                                   reinterpret_cast<const int64_t*>(base + c.prefix),
+                                   reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
                 break;
             case DIAG:
                 launch_permute_from_f64(reinterpret_cast<const PermDev*>(base + c.tasks),
@@ -341,17 +383,17 @@ void Batch::run(hipStream_t st) {
             case FILL:
                 launch_fill(reinterpret_cast<const FillDev*>(base + c.tasks),
                             reinterpret_cast<const int64_t*>(base + c.prefix),
-                            reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, dtype_, st);
+                            reinterpret_cast<const int32_t*>(base + c.wg), c.nwg, c.dt >= 0 ? c.dt : dtype_, st);
                 break;
             case GEMM:
                 launch_gemm(reinterpret_cast<const GemmDev*>(base + c.tasks),
-                            reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, dtype_, st);
+                            reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, c.dt >= 0 ? c.dt : dtype_, st);
                 break;
             case QR:
                 launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, c.lds, dtype_, st);
                 break;
             case SVD:
-                launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, c.lds, dtype_, st);
+                launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, c.lds, c.rpt, dtype_, st);
                 break;
         }
         if (timer_) {

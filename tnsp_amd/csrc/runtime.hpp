@@ -75,6 +75,13 @@ class Batch {
     void flush_fill();
     void gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es);
     void flush_gemm();
+    // reduced-precision contraction pieces (float64 storage, float32 GEMM
+    // operands): converting permutes, float32 fills and a GEMM launch in
+    // precision `prec` (1 fp32, 2 split-bf16 x3, 3 split-bf16 x6)
+    void perm_cvt(const std::vector<PermTask>& ts, void* const* bufs, int64_t es_src, int64_t es_dst);
+    void flush_perm_cvt(int conv);  // 1: float64 -> float32, 2: float32 -> float64
+    void flush_fill_as(int dt);
+    void flush_gemm_as(int prec);
     void qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, double* scratch);
     void flush_qr();
     void svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch);
@@ -92,6 +99,9 @@ class Batch {
     void reset();
     void set_timer(KernelTimer* t) { timer_ = t; }
     void set_staging(Staging* s) { staging_ = s; }
+    // large SVD blocks may use the register-resident kernel, whose V is
+    // recovered from U (no orthonormal completion for zero singular values)
+    void set_fast_svd(bool on) { fast_svd_ = on; }
 
    private:
     enum Kind { PERM, FILL, GEMM, QR, SVD, DIAG };
@@ -101,6 +111,9 @@ class Batch {
         int64_t count, nwg;
         double units = 0;  // algorithmic flops or bytes of this launch
         int64_t lds = 0;   // dynamic LDS bytes (decompositions)
+        int rpt = 0;       // register tile of the large-block SVD kernel
+        int dt = -1;       // element type / GEMM precision override (-1: batch dtype)
+        int conv = 0;      // permute conversion (1: f64 -> f32, 2: f32 -> f64)
     };
     template<typename D>
     void flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& cost, int64_t lds_max,
@@ -124,6 +137,7 @@ class Batch {
     std::vector<double> pqr_cost_, psvd_cost_;
     int64_t perm_elems_ = 0;
     double* scratch_cursor_ = nullptr;
+    bool fast_svd_ = false;
 };
 
 }  // namespace tns

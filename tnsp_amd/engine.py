@@ -22,7 +22,9 @@ STAT_NAMES = ["plan_s", "exec_s", "total_s", "gemm_flops", "qr_flops", "svd_flop
 
 
 class AmplitudeEngine:
-    def __init__(self, lattice, threads=None, chunk=1024, lanes=1):
+    PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+
+    def __init__(self, lattice, threads=None, chunk=1024, lanes=1, precision="fp64"):
         spec = lattice.spec
         self.lattice = lattice  # keeps the site buffers alive
         self.spec = spec
@@ -40,6 +42,16 @@ class AmplitudeEngine:
         self.handle = h
         self.set_chunk(chunk)
         self.set_lanes(lanes)
+        self.set_precision(precision)
+
+    def set_precision(self, precision):
+        """GEMM precision of the contractions: "fp64", "fp32", "bf16x3" or
+        "bf16x6" (split-bf16 products with fp32 accumulation).  QR, SVD and
+        all stored tensors stay float64."""
+        if precision not in self.PRECISIONS:
+            raise ValueError(f"precision must be one of {list(self.PRECISIONS)}")
+        _lib.check(_lib.lib().tns_engine_set_precision(self.handle, self.PRECISIONS[precision]))
+        self.precision = precision
 
     def set_lanes(self, lanes):
         """Number of concurrent lockstep lanes (own host thread + stream)."""
