@@ -39,6 +39,13 @@ def parse():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", type=int, default=0, help="also time N configs on the plain per-config path")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
+                    help="contraction GEMM precision of the timed engine")
+    ap.add_argument("--ess", default="", help="also measure ESS/sec of plain vs learned sampling on this workload")
+    ap.add_argument("--ess-chains", type=int, default=256)
+    ap.add_argument("--ess-samples", type=int, default=16)
+    ap.add_argument("--ess-train-sweeps", type=int, default=6)
+    ap.add_argument("--ess-epochs", type=int, default=6)
     return ap.parse_args()
 
 
@@ -105,6 +112,48 @@ def cpu_baseline(spec, storages, configs, per_proc, procs):
                       f"in-process, max over processes; wall incl. parsing {wall:.1f}s"}
 
 
+def ess_comparison(args, log):
+    """Plain sweep sampling vs learned-proposal Metropolis-Hastings on the
+    tV model (t = V = 1): ESS of the energy estimator per second of sampling.
+    The learned proposal is trained first on hops the plain sampler tried;
+    one learned sample is (bonds / 2) proposals per chain, about the
+    amplitude count of one plain sweep."""
+    import numpy as np
+
+    from tnsp_amd import peps
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.proposal import HopProposalNet, LearnedSampler, collect_training_data, train
+    from tnsp_amd.sampler import TVSampler, measure_ess
+
+    spec = getattr(peps, args.ess)
+    lat = peps.Lattice(spec)
+    eng = AmplitudeEngine(lat, chunk=1024, lanes=args.lanes)
+    L = spec.L1 * spec.L2
+    cand = peps.sample_configurations(spec, 4 * args.ess_chains, seed=77)
+    psi = eng.evaluate(cand)
+    start = cand[psi != 0][:args.ess_chains]
+    plain = TVSampler(eng, start, seed=1)
+    t0 = time.time()
+    occs, hops, targets = collect_training_data(plain, args.ess_train_sweeps)
+    t_collect = time.time() - t0
+    net = HopProposalNet(spec.L1, spec.L2, channels=32, layers=4, seed=3)
+    t0 = time.time()
+    losses = train(net, occs, hops, targets, epochs=args.ess_epochs, batch=1024, lr=2e-3, log=log)
+    t_train = time.time() - t0
+    res_plain = measure_ess(plain.sweep, plain.local_energy, args.ess_samples, sites=L)
+    res_plain["acceptance"] = plain.accepted / max(1, plain.proposals)
+    learned = LearnedSampler(eng, net, plain.configs.copy(), seed=2)
+    K = max(1, len(plain.bonds) // 2)
+    res_learned = measure_ess(lambda: [learned.step() for _ in range(K)],
+                              lambda: plain.local_energy(learned.configs, learned.psi), args.ess_samples, sites=L)
+    res_learned["acceptance"] = learned.accepted / max(1, learned.proposals)
+    res_learned["proposals_per_sample"] = K
+    return {"workload": args.ess, "chains": len(start), "plain": res_plain, "learned": res_learned,
+            "gain": res_learned["ess_per_s"] / max(res_plain["ess_per_s"], 1e-12),
+            "training": {"samples": int(len(hops)), "collect_s": t_collect, "train_s": t_train,
+                         "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -121,7 +170,7 @@ def main():
     storages = peps.random_storages(spec)
     lat = peps.Lattice(spec, storages)
     threads = args.threads or int(os.environ.get("TNS_THREADS", min(16, os.cpu_count() or 1)))
-    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk, lanes=args.lanes)
+    eng = AmplitudeEngine(lat, threads=threads, chunk=args.chunk, lanes=args.lanes, precision=args.precision)
     B = args.batch
     total_steps = args.warmup + args.steps
     particles = (24, 40) if spec.kind == "FermiZ2" else None
@@ -162,6 +211,9 @@ def main():
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(spec, storages, configs, args.cpu_configs, procs)
     nonzero = float(np.mean(psi != 0))
+    ess = None
+    if args.ess:
+        ess = ess_comparison(args, lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
     line = {
         "metric": "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec",
         "value": value,
@@ -173,12 +225,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else f"f64 (contraction GEMMs {args.precision})",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {spec.kind} {spec.L1}x{spec.L2} PEPS, bond segments "
                                f"{list(spec.segments)} around tV charge backgrounds, Dc={spec.cut}, N={spec.N}, "
                                f"{B} configurations per step drawn uniformly at fixed particle number, "
-                               f"batched lockstep engine, fp64 throughout",
+                               f"batched lockstep engine, contraction GEMMs {args.precision}, QR/SVD fp64",
                    "batch": B, "threads": threads, "lanes": args.lanes},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -188,7 +240,7 @@ def main():
                   "gemm_gflop": st["gemm_flops"] / 1e9, "qr_gflop": st["qr_flops"] / 1e9,
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
-                  "plain_path_configs_per_s": plain},
+                  "plain_path_configs_per_s": plain, "precision": args.precision, "ess": ess},
     }
     print(json.dumps(line), flush=True)
 

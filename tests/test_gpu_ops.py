@@ -149,3 +149,30 @@ def test_qr_matches_oracle(kind, direction, free):
     same_structure(gr, r)
     back = gq.contract(gr, {("cq", "cr")}).transpose(C.names)
     assert rel(storage(back), C.data) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["FermiU1", "BoseU1"])
+@pytest.mark.parametrize("direction", ["r", "q"])
+def test_qr_large_blocks(kind, direction):
+    """Sectors too large for LDS (tall 1100 x 150 and wide) take the blocked
+    Householder path: Q R reconstructs the input and Q is orthonormal."""
+    from oracle import tat
+
+    K = tat.KINDS[kind]
+    rng = np.random.default_rng(7)
+    a = tat.Tensor(K, ["a", "b", "c"], [([(0, 700), (1, 400)], False), ([(0, 3), (-1, 2)], True),
+                                        ([(0, 50), (1, 100), (-1, 30)], False)])
+    a.data[:] = rng.standard_normal(a.size)
+    g = to_gpu(a)
+    q, r = a.qr(direction, {"c"}, "cq", "cr")
+    gq, gr = g.qr(direction, {"c"}, "cq", "cr")
+    same_structure(gq, q)
+    same_structure(gr, r)
+    back = gq.contract(gr, {("cq", "cr")}).transpose(a.names)
+    assert rel(storage(back), a.data) < 1e-12
+    # Q^dagger Q is the identity on the common edge (up to fermion signs)
+    keep = [n for n in q.names if n != "cq"]
+    prod = gq.conjugate().edge_rename({"cq": "cq'"}).contract(gq, {(n, n) for n in keep})
+    ident = q.conjugate().edge_rename({"cq": "cq'"}).contract(q, {(n, n) for n in keep})
+    same_structure(prod, ident)
+    np.testing.assert_allclose(np.abs(storage(prod)), np.abs(ident.data), atol=1e-12)

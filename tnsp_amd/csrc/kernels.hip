@@ -415,6 +415,257 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_glb(const QrDev* __restrict__
     qr_body<T>(tk, tk.work, tk.work + int64_t(mx) * nx, red, bc);
 }
 
+// Blocked Householder QR for blocks that do not fit LDS (X in global
+// scratch, column-major, as in qr_body).  Panels of nb <= 16 columns are
+// factored in LDS (same reflector convention as qr_body), then the panel's
+// block reflector H = I - V T V^T (T from the LAPACK forward / columnwise
+// recurrence) is applied to the trailing columns, 32 at a time, as
+// W = V^T A, W = T^T W, A -= V W: three passes over the trailing matrix per
+// panel instead of two per column.  Q is formed backwards block by block
+// (A -= V (T (V^T A)) on identity-initialised columns).
+constexpr int QB_NB = 16, QB_CW = 32;
+constexpr int64_t QB_EXTRA = (QB_NB * QB_NB * 2 + QB_NB * QB_CW * 2 + QB_NB) * 8;
+
+__device__ inline int qb_panel_width(int mx) {
+    const int64_t vb = QB_LDS - QB_EXTRA;
+    const int64_t w = vb / (8 * int64_t(mx));
+    return int(w < QB_NB ? w : QB_NB);
+}
+
+// T (nb x nb, upper) of H = I - V T V^T from the unit-lower V (rows x nb,
+// column-major) and taus; Gm scratch nb x nb.  One wave owns row j of T.
+__device__ inline void qb_build_t(const double* V, int rows, int nb, const double* taus, double* Tm, double* Gm) {
+    const int tid = threadIdx.x;
+    // Gram entries Gm[l][i] = v_l . v_i for l < i, 4 lanes per pair
+    const int npair = nb * (nb - 1) / 2;
+    for (int base = 0; base < npair * 4; base += QR_WG) {
+        const int e = base + tid;
+        double acc = 0.0;
+        int l = -1, i = -1;
+        if (e < npair * 4) {
+            int q = e / 4;
+            i = 1;
+            while (q >= i) {
+                q -= i;
+                i++;
+            }
+            l = q;
+            for (int r = i + (e % 4); r < rows; r += 4) acc += V[int64_t(l) * rows + r] * V[int64_t(i) * rows + r];
+        }
+        acc += __shfl_xor(acc, 1, WAVE);
+        acc += __shfl_xor(acc, 2, WAVE);
+        if (e < npair * 4 && e % 4 == 0) Gm[l * QB_NB + i] = acc;
+    }
+    __syncthreads();
+    if (tid < nb) {
+        const int j = tid;
+        for (int i = 0; i < nb; i++) {
+            double v;
+            if (i < j) {
+                v = 0.0;
+            } else if (i == j) {
+                v = taus[i];
+            } else {
+                double sacc = 0.0;
+                for (int l = j; l < i; l++) sacc += Tm[j * QB_NB + l] * Gm[l * QB_NB + i];
+                v = -taus[i] * sacc;
+            }
+            Tm[j * QB_NB + i] = v;
+        }
+    }
+    __syncthreads();
+}
+
+// A (rows p0.., columns [c_begin, c_end) of X) -= V op(T) V^T A with
+// op(T) = T^T when trans (applying H^T) and T otherwise (applying H).
+__device__ inline void qb_apply(const double* V, int rows, int nb, const double* Tm, double* W, double* W2, double* X,
+                                int mx, int p0, int c_begin, int c_end, bool trans) {
+    const int tid = threadIdx.x, cc = tid / 16, rl = tid % 16;
+    for (int c0 = c_begin; c0 < c_end; c0 += QB_CW) {
+        const int cw = min(QB_CW, c_end - c0);
+        double acc[QB_NB];
+#pragma unroll
+        for (int j = 0; j < QB_NB; j++) acc[j] = 0.0;
+        if (cc < cw) {
+            const double* a = X + int64_t(c0 + cc) * mx + p0;
+            for (int r = rl; r < rows; r += 16) {
+                const double av = a[r];
+#pragma unroll
+                for (int j = 0; j < QB_NB; j++)
+                    if (j < nb) acc[j] += V[int64_t(j) * rows + r] * av;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < QB_NB; j++) {
+            double v = acc[j];
+            v += __shfl_xor(v, 1, WAVE);
+            v += __shfl_xor(v, 2, WAVE);
+            v += __shfl_xor(v, 4, WAVE);
+            v += __shfl_xor(v, 8, WAVE);
+            acc[j] = v;
+        }
+        if (rl == 0 && cc < cw)
+            for (int j = 0; j < nb; j++) W[j * QB_CW + cc] = acc[j];
+        __syncthreads();
+        {
+            const int j = tid % 16, c = tid / 16;
+            if (j < nb && c < cw) {
+                double sacc = 0.0;
+                if (trans) {
+                    for (int i = 0; i <= j; i++) sacc += Tm[i * QB_NB + j] * W[i * QB_CW + c];
+                } else {
+                    for (int i = j; i < nb; i++) sacc += Tm[j * QB_NB + i] * W[i * QB_CW + c];
+                }
+                W2[j * QB_CW + c] = sacc;
+            }
+        }
+        __syncthreads();
+        if (cc < cw) {
+            double* a = X + int64_t(c0 + cc) * mx + p0;
+            for (int r = rl; r < rows; r += 16) {
+                double sacc = 0.0;
+                for (int j = 0; j < nb; j++) sacc += V[int64_t(j) * rows + r] * W2[j * QB_CW + cc];
+                a[r] -= sacc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restrict__ tasks) {
+    extern __shared__ double sm[];
+    __shared__ double red[QR_WG / WAVE];
+    __shared__ double bc[2];
+    const QrDev tk = tasks[blockIdx.x];
+    const int m = tk.m, n = tk.n, lq = tk.lq;
+    const int mx = lq ? n : m, nx = lq ? m : n, k = mx < nx ? mx : nx;
+    const int nbmax = qb_panel_width(mx);
+    if (nbmax < 1) return;  // the host routes only blocks with nbmax >= 4 here
+    double* Tm = sm;
+    double* Gm = Tm + QB_NB * QB_NB;
+    double* W = Gm + QB_NB * QB_NB;
+    double* W2 = W + QB_NB * QB_CW;
+    double* taus = W2 + QB_NB * QB_CW;
+    double* V = taus + QB_NB;
+    double* X = tk.work;
+    double* tau = X + int64_t(mx) * nx;
+    const T* in = static_cast<const T*>(tk.in);
+    const int tid = threadIdx.x;
+    const int G = 32, NG = QR_WG / G, g = tid / G, gl = tid % G;
+    for (int64_t e = tid; e < int64_t(mx) * nx; e += QR_WG) {
+        const int r = int(e % mx), c = int(e / mx);
+        X[e] = double(lq ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
+    }
+    __syncthreads();
+    for (int p0 = 0; p0 < k; p0 += nbmax) {
+        const int nb = min(nbmax, k - p0), rows = mx - p0;
+        for (int64_t e = tid; e < int64_t(rows) * nb; e += QR_WG) {
+            const int c = int(e / rows), r = int(e % rows);
+            V[e] = X[int64_t(p0 + c) * mx + p0 + r];
+        }
+        __syncthreads();
+        for (int jj = 0; jj < nb; jj++) {
+            double* cj = V + int64_t(jj) * rows;
+            double sq = 0;
+            for (int r = jj + 1 + tid; r < rows; r += QR_WG) sq += cj[r] * cj[r];
+            const double sigma = block_sum<QR_WG>(sq, red);
+            if (tid == 0) {
+                const double alpha = cj[jj];
+                if (sigma == 0.0) {
+                    bc[0] = 0.0;
+                    bc[1] = alpha;
+                } else {
+                    const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+                    bc[0] = (beta - alpha) / beta;
+                    bc[1] = beta;
+                }
+            }
+            __syncthreads();
+            const double tj = bc[0], beta = bc[1];
+            if (tj != 0.0) {
+                const double scale = 1.0 / (cj[jj] - beta);
+                for (int r = jj + 1 + tid; r < rows; r += QR_WG) cj[r] *= scale;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                cj[jj] = beta;
+                taus[jj] = tj;
+                tau[p0 + jj] = tj;
+            }
+            if (tj != 0.0) {
+                for (int c = jj + 1 + g; c < nb; c += NG) {
+                    double* cc = V + int64_t(c) * rows;
+                    double d = 0;
+                    for (int r = jj + 1 + gl; r < rows; r += G) d += cj[r] * cc[r];
+                    d = group_sum(d, G) + cc[jj];
+                    const double f = tj * d;
+                    if (gl == 0) cc[jj] -= f;
+                    for (int r = jj + 1 + gl; r < rows; r += G) cc[r] -= f * cj[r];
+                }
+            }
+            __syncthreads();
+        }
+        // panel back to X (R on and above the diagonal, reflectors below),
+        // then V as unit lower for the block reflector
+        for (int64_t e = tid; e < int64_t(rows) * nb; e += QR_WG) {
+            const int c = int(e / rows), r = int(e % rows);
+            const double v = V[e];
+            X[int64_t(p0 + c) * mx + p0 + r] = v;
+            V[e] = r < c ? 0.0 : (r == c ? 1.0 : v);
+        }
+        __syncthreads();
+        if (p0 + nb < nx) {
+            qb_build_t(V, rows, nb, taus, Tm, Gm);
+            qb_apply(V, rows, nb, Tm, W, W2, X, mx, p0, p0 + nb, nx, true);
+        }
+        __syncthreads();
+    }
+    T* f1 = static_cast<T*>(tk.f1);
+    T* f2 = static_cast<T*>(tk.f2);
+    if (!lq) {
+        for (int64_t e = tid; e < int64_t(k) * n; e += QR_WG) {
+            const int r = int(e / n), c = int(e % n);
+            f2[e] = T(r <= c ? X[int64_t(c) * mx + r] : 0.0);
+        }
+    } else {
+        for (int64_t e = tid; e < int64_t(m) * k; e += QR_WG) {
+            const int i = int(e / k), r = int(e % k);
+            f1[e] = T(r <= i ? X[int64_t(i) * mx + r] : 0.0);
+        }
+    }
+    __syncthreads();
+    // Q = H_1 ... H_k I[:, :k], last panel first
+    for (int p0 = ((k - 1) / nbmax) * nbmax; p0 >= 0; p0 -= nbmax) {
+        const int nb = min(nbmax, k - p0), rows = mx - p0;
+        for (int64_t e = tid; e < int64_t(rows) * nb; e += QR_WG) {
+            const int c = int(e / rows), r = int(e % rows);
+            V[e] = r < c ? 0.0 : (r == c ? 1.0 : X[int64_t(p0 + c) * mx + p0 + r]);
+        }
+        if (tid < nb) taus[tid] = tau[p0 + tid];
+        __syncthreads();
+        for (int64_t e = tid; e < int64_t(mx) * nb; e += QR_WG) {
+            const int c = int(e / mx), r = int(e % mx);
+            X[int64_t(p0 + c) * mx + r] = (r == p0 + c) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        qb_build_t(V, rows, nb, taus, Tm, Gm);
+        qb_apply(V, rows, nb, Tm, W, W2, X, mx, p0, p0, k, false);
+        __syncthreads();
+    }
+    if (!lq) {
+        for (int64_t e = tid; e < int64_t(m) * k; e += QR_WG) {
+            const int r = int(e / k), c = int(e % k);
+            f1[e] = T(X[int64_t(c) * mx + r]);
+        }
+    } else {
+        for (int64_t e = tid; e < int64_t(k) * n; e += QR_WG) {
+            const int r = int(e / n), i = int(e % n);
+            f2[e] = T(X[int64_t(r) * mx + i]);
+        }
+    }
+}
+
 // -------------------------------------------------------------------- svd
 // Ranks the singular values (descending, ties by column) and writes S, U
 // and V^T of the block from X (normalised columns), V and sig.
@@ -884,9 +1135,19 @@ void allow_lds(K kernel, int64_t bytes) {
 }
 }  // namespace
 
-void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream) {
+void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (lds_bytes > 0) {
+    if (variant == 1) {
+        if (dtype == 0) {
+            allow_lds(qr_kernel_blocked<double>, QB_LDS);
+            hipLaunchKernelGGL(qr_kernel_blocked<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
+                               tasks);
+        } else {
+            allow_lds(qr_kernel_blocked<float>, QB_LDS);
+            hipLaunchKernelGGL(qr_kernel_blocked<float>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
+                               tasks);
+        }
+    } else if (lds_bytes > 0) {
         if (dtype == 0) {
             allow_lds(qr_kernel_lds<double>, lds_bytes);
             hipLaunchKernelGGL(qr_kernel_lds<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(lds_bytes), stream, tasks);

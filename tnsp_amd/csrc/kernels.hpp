@@ -91,7 +91,13 @@ inline int64_t svd_doubles(int64_t m, int64_t n) {
     const int64_t mx = m > n ? m : n, kx = m < n ? m : n;
     return mx * kx + kx * kx + kx;
 }
-void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int dtype, hipStream_t stream);
+// variant 1: blocked Householder with the panel in LDS (global scratch
+// blocks with at most QB_MAX_ROWS rows); otherwise LDS (lds_bytes > 0) or
+// unblocked global scratch
+void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream);
+constexpr int64_t QB_LDS = 156 * 1024;
+// rows for which the blocked kernel's panel keeps at least 4 columns
+constexpr int64_t QB_MAX_ROWS = (QB_LDS - 8 * (16 * 16 * 2 + 16 * 32 * 2 + 16)) / (8 * 4);
 // rpt > 0: register-resident kernel for blocks with at most 8 * rpt rows
 // (rpt in {8, 16, 24}), V recovered from U; otherwise LDS (lds_bytes > 0)
 // or global scratch.

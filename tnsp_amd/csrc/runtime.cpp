@@ -284,17 +284,20 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
     // part 0: global scratch, parts 1..3: LDS classes (largest first),
-    // parts 4..6: register-resident SVD with 24 / 16 / 8 rows per lane
-    std::vector<D> parts[7];
-    double units[7] = {0, 0, 0, 0, 0, 0, 0};
-    int64_t need[7] = {0, 0, 0, 0, 0, 0, 0};
-    static const int part_rpt[7] = {0, 0, 0, 0, 24, 16, 8};
+    // parts 4..6: register-resident SVD with 24 / 16 / 8 rows per lane,
+    // part 7: blocked QR
+    std::vector<D> parts[8];
+    double units[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t need[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static const int part_rpt[8] = {0, 0, 0, 0, 24, 16, 8, 1};
     for (size_t i : order) {
         const int64_t bytes = doubles(tasks[i].m, tasks[i].n) * 8;
         const int64_t rows = std::max<int64_t>(tasks[i].m, tasks[i].n);
         int p = 0;
         if (kind == SVD && fast_svd_ && bytes > LDS_CLASS[1] && rows <= SVD_REG_MAX_ROWS) {
             p = rows <= 64 ? 6 : (rows <= 128 ? 5 : 4);
+        } else if (kind == QR && bytes > lds_max && rows <= QB_MAX_ROWS) {
+            p = 7;
         } else if (bytes <= lds_max) {
             p = bytes <= LDS_CLASS[0] ? 3 : (bytes <= LDS_CLASS[1] ? 2 : 1);
         }
@@ -302,7 +305,7 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
         units[p] += cost[i];
         need[p] = std::max(need[p], bytes);
     }
-    for (int part = 0; part < 7; part++) {
+    for (int part = 0; part < 8; part++) {
         auto& v = parts[part];
         if (v.empty()) continue;
         Cmd c;
@@ -311,7 +314,7 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
         c.prefix = c.wg = 0;
         c.count = c.nwg = int64_t(v.size());
         c.units = units[part];
-        c.lds = (part == 0 || part >= 4) ? 0 : (need[part] + 255) / 256 * 256;
+        c.lds = (part == 0 || part >= 4) ? 0 : (need[part] + 255) / 256 * 256;  // fixed inside the launcher
         c.rpt = part_rpt[part];
         cmds_.push_back(c);
     }
@@ -390,7 +393,7 @@ void Batch::run(hipStream_t st) {
                             reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, c.dt >= 0 ? c.dt : dtype_, st);
                 break;
             case QR:
-                launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, c.lds, dtype_, st);
+                launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, c.lds, c.rpt, dtype_, st);
                 break;
             case SVD:
                 launch_svd(reinterpret_cast<const SvdDev*>(base + c.tasks), c.count, c.lds, c.rpt, dtype_, st);

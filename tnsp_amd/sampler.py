@@ -183,25 +183,26 @@ def integrated_autocorrelation(series, c=5.0):
     return max(tau, 1e-9)
 
 
-def measure_ess(sampler, sweeps, burn_in=2):
-    """Runs ``burn_in`` + ``sweeps`` sweeps, recording E_loc after each;
-    returns (ESS, sampling seconds, energy per site, tau, acceptance)."""
+def measure_ess(advance, energy, samples, burn_in=2, sites=1):
+    """Calls ``advance()`` ``burn_in`` + ``samples`` times, recording
+    ``energy()`` (per-chain local energies) after each timed advance; returns
+    ESS of the energy estimator, the sampling seconds (energy evaluation not
+    included), the energy per site and tau."""
     for _ in range(burn_in):
-        sampler.sweep()
+        advance()
     energies = []
     t_sample = 0.0
-    for _ in range(sweeps):
+    for _ in range(samples):
         t0 = time.time()
-        sampler.sweep()
+        advance()
         t_sample += time.time() - t0
-        energies.append(sampler.local_energy())
+        energies.append(energy())
     series = np.stack(energies, axis=1)
     tau = integrated_autocorrelation(series)
     ess = series.size / tau
-    L = sampler.L1 * sampler.L2
-    acc = sampler.accepted / max(1, sampler.proposals)
-    return {"ess": ess, "sample_s": t_sample, "energy_per_site": float(series.mean() / L), "tau": tau,
-            "acceptance": acc, "chains": series.shape[0], "sweeps": sweeps}
+    return {"ess": ess, "sample_s": t_sample, "ess_per_s": ess / max(t_sample, 1e-12),
+            "energy_per_site": float(series.mean() / sites), "tau": tau, "chains": series.shape[0],
+            "samples": samples}
 
 
 def make_engine(lattice, **kw):
