@@ -49,6 +49,18 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of ``kernel`` from the committed FETCH_SIZE /
+    WRITE_SIZE profile of the same workload and launch size (rocprofv3 --pmc
+    runs scaled by the counters' reading for a device copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "c4_traffic.json")
+    if args.workload != "C4" or args.chunk != 256 or args.precision != "fp64" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return k["traffic_bytes"] if k else None
+
+
 def dominant_kernel(kstats):
     best = max(kstats.items(), key=lambda kv: kv[1]["ms"])
     kind, s = best
@@ -199,6 +211,8 @@ def main():
     value = B * args.steps / elapsed
     dom = dominant_kernel(ks)
     roof = dom[1] if dom else None
+    if roof:
+        roof["traffic"] = measured_traffic(dom[0], args)
     plain = None
     if args.plain:
         t1 = time.time()

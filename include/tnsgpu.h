@@ -116,9 +116,10 @@ int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations
  * launches device_bytes configs (n <= 10) */
 int tns_engine_stats(const tns_engine* e, double* values, int n);
 int tns_engine_reset_stats(tns_engine* e);
-/* per-kernel-kind device timing with HIP events on the launch stream:
- * kinds permute, fill, gemm, qr, svd, diag; per kind: launches, total ms,
- * algorithmic work (bytes for permute/fill/diag, flops for gemm/qr/svd) */
+/* per-kernel device timing with HIP events on the launch stream: permute,
+ * fill, gemm, qr (LDS / unblocked), svd (LDS / global), diag, qr_blocked,
+ * svd_reg8, svd_reg16, svd_reg24; per kernel: launches, total ms,
+ * algorithmic work (bytes for permute/fill/diag, flops for the others) */
 int tns_engine_set_profile(tns_engine* e, int on);
 int tns_engine_kernel_stats(const tns_engine* e, double* values, int n);
 int tns_engine_free(tns_engine* e);

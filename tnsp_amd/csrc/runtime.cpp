@@ -401,7 +401,12 @@ void Batch::run(hipStream_t st) {
         }
         if (timer_) {
             check_hip(hipEventRecord(eb, st), "timer record");
-            timer_->add(int(c.kind), ea, eb, c.units);
+            int k = int(c.kind);
+            if (c.kind == QR && c.rpt == 1) k = KernelTimer::QR_BLOCKED;
+            if (c.kind == SVD && c.rpt == 8) k = KernelTimer::SVD_REG8;
+            if (c.kind == SVD && c.rpt == 16) k = KernelTimer::SVD_REG16;
+            if (c.kind == SVD && c.rpt == 24) k = KernelTimer::SVD_REG24;
+            timer_->add(k, ea, eb, c.units);
         }
     }
     reset();

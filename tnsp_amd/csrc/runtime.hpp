@@ -42,7 +42,9 @@ class Staging {
 // launch stream, plus the algorithmic work of each launch (flops for gemm /
 // qr / svd, bytes moved for permute / fill).
 struct KernelTimer {
-    enum { PERM, FILL, GEMM, QR, SVD, DIAG, NKIND };
+    // one entry per kernel (variant) so that launch averages compare with a
+    // kernel trace: QR / SVD are the LDS and unblocked-global kernels
+    enum { PERM, FILL, GEMM, QR, SVD, DIAG, QR_BLOCKED, SVD_REG8, SVD_REG16, SVD_REG24, NKIND };
     struct Acc {
         double ms = 0, units = 0, launches = 0;
     } acc[NKIND];

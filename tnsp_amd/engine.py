@@ -75,16 +75,17 @@ class AmplitudeEngine:
         _lib.check(_lib.lib().tns_engine_stats(self.handle, vals, len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, list(vals)))
 
-    KERNEL_KINDS = ["permute", "fill", "gemm", "qr", "svd", "diag"]
+    KERNEL_KINDS = ["permute", "fill", "gemm", "qr", "svd", "diag", "qr_blocked", "svd_reg8", "svd_reg16", "svd_reg24"]
 
     def set_profile(self, on=True):
         _lib.check(_lib.lib().tns_engine_set_profile(self.handle, int(bool(on))))
 
     def kernel_stats(self):
-        """{kind: {"launches", "ms", "units"}}; units are bytes for permute /
-        fill / diag and flops for gemm / qr / svd."""
-        vals = (ctypes.c_double * 18)()
-        _lib.check(_lib.lib().tns_engine_kernel_stats(self.handle, vals, 18))
+        """{kernel: {"launches", "ms", "units"}}; units are bytes for permute /
+        fill / diag and flops for gemm and the QR / SVD kernels."""
+        n = 3 * len(self.KERNEL_KINDS)
+        vals = (ctypes.c_double * n)()
+        _lib.check(_lib.lib().tns_engine_kernel_stats(self.handle, vals, n))
         return {k: {"launches": vals[3 * i], "ms": vals[3 * i + 1], "units": vals[3 * i + 2]}
                 for i, k in enumerate(self.KERNEL_KINDS)}
 
