@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--plain", type=int, default=0, help="also time N configs on the plain per-config path")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="contraction GEMM precision of the timed engine")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile per size class")
     ap.add_argument("--ess", default="", help="also measure ESS/sec of plain vs learned sampling on this workload")
     ap.add_argument("--ess-chains", type=int, default=256)
     ap.add_argument("--ess-samples", type=int, default=16)
@@ -189,6 +190,13 @@ def main():
     configs = peps.sample_configurations(spec, B * total_steps, seed=1234, particles=particles)
     print(f"[bench] workload {args.workload} batch {B} threads {threads} lanes {args.lanes}", file=sys.stderr,
           flush=True)
+    tuned = None
+    if not args.no_autotune and args.precision in ("fp64", "fp32"):
+        t0 = time.time()
+        table, ms = eng.autotune(configs[:min(B, args.chunk)])
+        tuned = {"table": table, "ms": ms.round(3).tolist()}
+        print(f"[bench] GEMM tiles per class (<=16, <=48, larger): {table} ({time.time() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
     for w in range(args.warmup):
         t0 = time.time()
         eng.evaluate(configs[w * B:(w + 1) * B])
@@ -254,7 +262,7 @@ def main():
                   "gemm_gflop": st["gemm_flops"] / 1e9, "qr_gflop": st["qr_flops"] / 1e9,
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
-                  "plain_path_configs_per_s": plain, "precision": args.precision, "ess": ess},
+                  "plain_path_configs_per_s": plain, "precision": args.precision, "gemm_tiles": tuned, "ess": ess},
     }
     print(json.dumps(line), flush=True)
 

@@ -110,6 +110,15 @@ int tns_engine_set_lanes(tns_engine* e, int lanes);
  * and all stored tensors stay float64.  No reference counterpart (TAT
  * computes in the tensor's own scalar type). */
 int tns_engine_set_precision(tns_engine* e, int precision);
+/* GEMM tile autotuner (no reference counterpart): evaluates the given
+ * configurations on lane 0 and, at every contraction of that evaluation,
+ * times each GEMM tile variant (0: 32x32, 1: 64x64, 2: 16x16) on the
+ * contraction's own sector GEMMs for each size class (max(m, n) <= 16,
+ * <= 48, larger); the variant with the least total time per class is used
+ * by every lane from then on.  table (3) gets the choice, ms (9,
+ * class-major) the summed kernel times; either may be NULL. */
+int tns_engine_autotune(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
+                        double relative_cut, int32_t* table, double* ms, void* stream);
 int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
                         double relative_cut, double* psi, void* stream);
 /* values: plan_s exec_s total_s gemm_flops qr_flops svd_flops perm_elems

@@ -176,3 +176,22 @@ def test_qr_large_blocks(kind, direction):
     ident = q.conjugate().edge_rename({"cq": "cq'"}).contract(q, {(n, n) for n in keep})
     same_structure(prod, ident)
     np.testing.assert_allclose(np.abs(storage(prod)), np.abs(ident.data), atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_contract_every_gemm_tile(dtype):
+    """Sectors of 3 .. 130 rows / columns exercise the 16x16, 32x32 and 64x64
+    GEMM tiles (size classes <= 16, <= 48, larger)."""
+    from oracle import tat
+
+    K = tat.KINDS["FermiU1"]
+    rng = np.random.default_rng(11)
+    seg = [(-1, 3), (0, 40), (1, 130)]
+    A = tat.Tensor(K, ["i", "k"], [(seg, False), (seg, True)])
+    A.data[:] = rng.standard_normal(A.size)
+    B = tat.Tensor(K, ["k", "j"], [A.edges[1].conjugate(), (seg, True)])
+    B.data[:] = rng.standard_normal(B.size)
+    C = A.contract(B, {("k", "k")})
+    gC = to_gpu(A, dtype).contract(to_gpu(B, dtype), {("k", "k")})
+    same_structure(gC, C)
+    assert rel(storage(gC), C.data) < (1e-12 if dtype == "float64" else 1e-5)

@@ -54,6 +54,7 @@ _SIGNATURES = {
     "tns_engine_set_chunk": [c_vp, ctypes.c_int64],
     "tns_engine_set_lanes": [c_vp, ctypes.c_int],
     "tns_engine_set_precision": [c_vp, ctypes.c_int],
+    "tns_engine_autotune": [c_vp, ctypes.c_int64, c_i32p, ctypes.c_int64, ctypes.c_double, c_i32p, c_dblp, c_vp],
     "tns_engine_evaluate": [c_vp, ctypes.c_int64, c_i32p, ctypes.c_int64, ctypes.c_double, c_dblp, c_vp],
     "tns_engine_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_reset_stats": [c_vp],

@@ -188,7 +188,57 @@ void Engine::contract(std::vector<Slot*>& a, std::vector<Slot*>& b,
     }
     grow_all(st);
     t0 = now();
+    if (tune_pending_ && !lowp) {
+        // operands first, then every variant on each class's GEMMs (3 timed
+        // repetitions after one warm-up); the GEMMs only write the product
+        Batch prep(dtype_);
+        char* Tp = arenas_[A_TEMP].base;
+        for (int64_t i = 0; i < B; i++) {
+            void* bufs[6] = {ptr(*a[i]), ptr(*b[i]), Tp + ta[i], Tp + tb[i], Tp + tp[i], nullptr};
+            prep.perm(plans[i].perm_a, bufs, es_);
+            prep.perm(plans[i].perm_b, bufs, es_);
+        }
+        prep.flush_perm();
+        for (int64_t i = 0; i < B; i++) {
+            void* bufs[6] = {nullptr, nullptr, Tp + ta[i], Tp + tb[i], Tp + tp[i], nullptr};
+            prep.fill(plans[i].fills, bufs, es_);
+        }
+        prep.flush_fill();
+        run_batch(prep, st);
+        for (int c = 0; c < 3; c++) {
+                for (int v = 0; v < GEMM_VARIANTS; v++) {
+                const int table[3] = {v, v, v};
+                KernelTimer tm;
+                double ms = 0;
+                bool any = false;
+                for (int rep = 0; rep < 4; rep++) {
+                    Batch g(dtype_);
+                    g.set_gemm_table(table);
+                    g.set_staging(&staging_);
+                    if (rep > 0) g.set_timer(&tm);
+                    for (int64_t i = 0; i < B; i++) {
+                        void* bufs[6] = {nullptr, nullptr, Tp + ta[i], Tp + tb[i], Tp + tp[i], nullptr};
+                        std::vector<GemmTask> sel;
+                        for (auto& t : plans[i].gemms)
+                            if (Batch::gemm_class(t.m, t.n) == c) sel.push_back(t);
+                        g.gemm(sel, bufs, es_);
+                    }
+                    g.flush_gemm();
+                    any = !g.empty();
+                    if (!any) break;
+                    g.run(st);
+                }
+                if (!any) break;
+                check_hip(hipStreamSynchronize(st), "tuning sync");
+                tm.collect();
+                ms = tm.acc[KernelTimer::GEMM].ms / 3.0;
+                tune_ms_[3 * c + v] += ms;
+            }
+        }
+    }
     Batch bt(dtype_);
+    static const int bf16_table[3] = {0, 0, 0};
+    bt.set_gemm_table(gemm_prec_ >= 2 ? bf16_table : gemm_table_);
     char* T = arenas_[A_TEMP].base;
     for (int64_t i = 0; i < B; i++) {
         void* bufs[6] = {ptr(*a[i]), ptr(*b[i]), T + ta[i], T + tb[i], T + tp[i], arenas_[oa].base + to[i]};
@@ -661,6 +711,7 @@ struct tns_engine {
     std::vector<int32_t> names;
     int64_t chunk = 1024;
     int precision = 0;
+    std::vector<int> gemm_table;  // tuned tile variants, empty until autotuned
     bool profile = false;
     std::vector<std::unique_ptr<tns::Engine>> lanes;
     std::vector<hipStream_t> streams;  // streams[0] unused (caller's stream)
@@ -675,6 +726,7 @@ struct tns_engine {
             lanes.push_back(std::make_unique<tns::Engine>(kind, L1, L2, sites, data, dtype, names, per));
             lanes.back()->set_chunk(chunk);
             lanes.back()->set_precision(precision);
+            if (gemm_table.size() == 3) lanes.back()->set_gemm_table(gemm_table.data());
             lanes.back()->set_profile(profile);
             hipStream_t s = nullptr;
             if (lanes.size() > 1) tns::check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lane stream");
@@ -719,6 +771,24 @@ int tns_engine_set_precision(tns_engine* e, int precision) {
         if (precision != 0 && e->dtype != 0) throw tns::Error("reduced-precision GEMMs need a float64 engine");
         e->precision = precision;
         for (auto& l : e->lanes) l->set_precision(precision);
+    })
+}
+
+int tns_engine_autotune(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
+                        double relative_cut, int32_t* table, double* ms, void* stream) {
+    TNS_TRY({
+        if (nconf <= 0) throw tns::Error("autotune needs configurations");
+        std::vector<double> psi(static_cast<size_t>(nconf), 0.0);
+        auto& l0 = *e->lanes[0];
+        l0.request_gemm_tuning();
+        l0.evaluate(nconf, occupations, remain_cut, relative_cut, psi.data(), static_cast<hipStream_t>(stream));
+        l0.finish_gemm_tuning();
+        for (auto& l : e->lanes) l->set_gemm_table(l0.gemm_table());
+        e->gemm_table.assign(l0.gemm_table(), l0.gemm_table() + 3);
+        if (table)
+            for (int c = 0; c < 3; c++) table[c] = l0.gemm_table()[c];
+        if (ms)
+            for (int i = 0; i < 9; i++) ms[i] = l0.gemm_tuning_ms()[i];
     })
 }
 

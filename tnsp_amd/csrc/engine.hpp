@@ -82,6 +82,28 @@ class Engine {
     // 2 split-bf16 x3, 3 split-bf16 x6 (QR, SVD and every stored tensor stay
     // float64)
     void set_precision(int p) { gemm_prec_ = dtype_ == 0 ? p : 0; }
+    // GEMM tile autotuner: until finish_gemm_tuning, every contraction times
+    // each tile variant on its own sector GEMMs, per size class; the sums
+    // over all contractions decide the variant of each class
+    void request_gemm_tuning() {
+        tune_pending_ = true;
+        for (auto& x : tune_ms_) x = 0;
+    }
+    void finish_gemm_tuning() {
+        tune_pending_ = false;
+        for (int c = 0; c < 3; c++) {
+            int best = -1;
+            for (int v = 0; v < 3; v++)
+                if (tune_ms_[3 * c + v] > 0 && (best < 0 || tune_ms_[3 * c + v] < tune_ms_[3 * c + best])) best = v;
+            if (best >= 0) gemm_table_[c] = best;
+        }
+    }
+    const int* gemm_table() const { return gemm_table_; }
+    void set_gemm_table(const int* t) {
+        for (int c = 0; c < 3; c++) gemm_table_[c] = t[c];
+    }
+    // kernel ms of every (class, variant) pair measured by the last tuning
+    const double* gemm_tuning_ms() const { return tune_ms_; }
     const EngineStats& stats() const { return stats_; }
     void reset_stats() {
         stats_ = EngineStats();
@@ -120,6 +142,9 @@ class Engine {
     Pool pool_;
     int64_t chunk_ = 1024;
     int gemm_prec_ = 0;
+    int gemm_table_[3] = {2, 0, 1};  // GEMM tile variant per sector size class
+    bool tune_pending_ = false;
+    double tune_ms_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t remain_cut_ = -1;
     double relative_cut_ = 0.0;
     EngineStats stats_;

@@ -122,29 +122,38 @@ __device__ inline void bf16_split(float x, s4* terms, int j) {
     }
 }
 
-// C tile 32x32 per workgroup; wave w owns the 16x16 sub-tile (w/2, w%2).
-// MFMA 16x16x4 operand layout: lane l holds A(l%16, l/16) and B(l/16, l%16);
-// accumulator rows as given by Mfma<T>::row.
-template<typename T>
-__global__ __launch_bounds__(WG) void gemm_kernel(const GemmDev* __restrict__ tasks, const TileDev* __restrict__ tiles) {
-    __shared__ T As[GEMM_TILE][KB + 1];
-    __shared__ T Bs[KB][GEMM_TILE + 1];
+// C tile (16 WR FM) x (16 WC FN) per workgroup of WR x WC waves; wave
+// (wr, wc) owns FM x FN fragments of 16x16.  MFMA 16x16x4 operand layout:
+// lane l holds A(l%16, l/16) and B(l/16, l%16); accumulator rows as given by
+// Mfma<T>::row.  Variants: 32x32 (2x2 waves, 1x1 fragments), 64x64 (2x2
+// waves, 2x2 fragments: each operand fragment feeds two MFMAs), 16x16 (one
+// wave, for sectors of at most 16 rows and columns).
+template<typename T, int WR, int WC, int FM, int FN>
+__global__ __launch_bounds__(64 * WR * WC) void gemm_kernel(const GemmDev* __restrict__ tasks,
+                                                           const TileDev* __restrict__ tiles) {
+    constexpr int NT = 64 * WR * WC, TM = 16 * WR * FM, TN = 16 * WC * FN;
+    __shared__ T As[TM][KB + 1];
+    __shared__ T Bs[KB][TN + 1];
     const TileDev tile = tiles[blockIdx.x];
     const GemmDev g = tasks[tile.task];
     const T* A = static_cast<const T*>(g.a);
     const T* B = static_cast<const T*>(g.b);
-    const int row0 = tile.tm * GEMM_TILE, col0 = tile.tn * GEMM_TILE;
+    const int row0 = tile.tm * TM, col0 = tile.tn * TN;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const int wr = (wave / 2) * 16, wc = (wave % 2) * 16;
-    auto acc = Mfma<T>::zero();
+    const int wr = (wave / WC) * 16 * FM, wc = (wave % WC) * 16 * FN;
+    typename Mfma<T>::acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::zero();
     for (int k0 = 0; k0 < g.k; k0 += KB) {
-        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+        for (int e = tid; e < TM * KB; e += NT) {
             // A tile: rows row0.., cols k0..; index so that consecutive threads
             // walk the contiguous direction of the stored operand
             int i, p;
             if (g.ta) {
-                i = e % GEMM_TILE;
-                p = e / GEMM_TILE;
+                i = e % TM;
+                p = e / TM;
             } else {
                 p = e % KB;
                 i = e / KB;
@@ -154,14 +163,14 @@ __global__ __launch_bounds__(WG) void gemm_kernel(const GemmDev* __restrict__ ta
             if (gi < g.m && gp < g.k) v = g.ta ? A[int64_t(gp) * g.lda + gi] : A[int64_t(gi) * g.lda + gp];
             As[i][p] = v;
         }
-        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+        for (int e = tid; e < TN * KB; e += NT) {
             int p, j;
             if (g.tb) {
                 p = e % KB;
                 j = e / KB;
             } else {
-                j = e % GEMM_TILE;
-                p = e / GEMM_TILE;
+                j = e % TN;
+                p = e / TN;
             }
             const int gp = k0 + p, gj = col0 + j;
             T v = T(0);
@@ -171,44 +180,55 @@ __global__ __launch_bounds__(WG) void gemm_kernel(const GemmDev* __restrict__ ta
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < KB; kk += 4) {
-            const T a = As[wr + lane % 16][kk + lane / 16];
-            const T b = Bs[kk + lane / 16][wc + lane % 16];
-            acc = Mfma<T>::step(a, b, acc);
+            T a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; i++) a[i] = As[wr + 16 * i + lane % 16][kk + lane / 16];
+#pragma unroll
+            for (int j = 0; j < FN; j++) b[j] = Bs[kk + lane / 16][wc + 16 * j + lane % 16];
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::step(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
     T* C = static_cast<T*>(g.c);
     const T alpha = T(g.alpha);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int r = row0 + wr + Mfma<T>::row(lane, i);
-        const int c = col0 + wc + lane % 16;
-        if (r < g.m && c < g.n) C[int64_t(r) * g.ldc + c] = alpha * acc[i];
-    }
+    for (int fi = 0; fi < FM; fi++)
+#pragma unroll
+        for (int fj = 0; fj < FN; fj++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = row0 + wr + 16 * fi + Mfma<T>::row(lane, i);
+                const int c = col0 + wc + 16 * fj + lane % 16;
+                if (r < g.m && c < g.n) C[int64_t(r) * g.ldc + c] = alpha * acc[fi][fj][i];
+            }
 }
 
-// Same tiling as gemm_kernel<float>, with each 16-deep k step done as
+// Same tiling as the 32x32 gemm_kernel<float>, with each 16-deep k step done as
 // split-bf16 MFMA products.  Lane l holds A(l % 16, 4 (l / 16) + j) and
 // B(4 (l / 16) + j, l % 16), j = 0..3; accumulator rows 4 (l / 16) + i.
 template<int NTERM>
 __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict__ tasks,
                                                        const TileDev* __restrict__ tiles) {
-    __shared__ float As[GEMM_TILE][KB + 1];
-    __shared__ float Bs[KB][GEMM_TILE + 1];
+    constexpr int BT = 32;
+    __shared__ float As[BT][KB + 1];
+    __shared__ float Bs[KB][BT + 1];
     const TileDev tile = tiles[blockIdx.x];
     const GemmDev g = tasks[tile.task];
     const float* A = static_cast<const float*>(g.a);
     const float* B = static_cast<const float*>(g.b);
-    const int row0 = tile.tm * GEMM_TILE, col0 = tile.tn * GEMM_TILE;
+    const int row0 = tile.tm * BT, col0 = tile.tn * BT;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const int wr = (wave / 2) * 16, wc = (wave % 2) * 16;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < g.k; k0 += KB) {
-        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+        for (int e = tid; e < BT * KB; e += WG) {
             int i, p;
             if (g.ta) {
-                i = e % GEMM_TILE;
-                p = e / GEMM_TILE;
+                i = e % BT;
+                p = e / BT;
             } else {
                 p = e % KB;
                 i = e / KB;
@@ -218,14 +238,14 @@ __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict
             if (gi < g.m && gp < g.k) v = g.ta ? A[int64_t(gp) * g.lda + gi] : A[int64_t(gi) * g.lda + gp];
             As[i][p] = v;
         }
-        for (int e = tid; e < GEMM_TILE * KB; e += WG) {
+        for (int e = tid; e < BT * KB; e += WG) {
             int p, j;
             if (g.tb) {
                 p = e % KB;
                 j = e / KB;
             } else {
-                j = e % GEMM_TILE;
-                p = e / GEMM_TILE;
+                j = e % BT;
+                p = e / BT;
             }
             const int gp = k0 + p, gj = col0 + j;
             float v = 0.f;
@@ -419,12 +439,12 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_glb(const QrDev* __restrict__
 // scratch, column-major, as in qr_body).  Panels of nb <= 16 columns are
 // factored in LDS (same reflector convention as qr_body), then the panel's
 // block reflector H = I - V T V^T (T from the LAPACK forward / columnwise
-// recurrence) is applied to the trailing columns, 32 at a time, as
-// W = V^T A, W = T^T W, A -= V W: three passes over the trailing matrix per
+// recurrence) is applied to the trailing columns on the fp64 MFMA as
+// W = V^T A, W = T^T W, A -= V W: two passes over the trailing matrix per
 // panel instead of two per column.  Q is formed backwards block by block
 // (A -= V (T (V^T A)) on identity-initialised columns).
-constexpr int QB_NB = 16, QB_CW = 32;
-constexpr int64_t QB_EXTRA = (QB_NB * QB_NB * 2 + QB_NB * QB_CW * 2 + QB_NB) * 8;
+constexpr int QB_NB = 16, QB_WC = 128;
+constexpr int64_t QB_EXTRA = (QB_NB * QB_NB * 2 + QB_NB * QB_WC + QB_NB) * 8;
 
 __device__ inline int qb_panel_width(int mx) {
     const int64_t vb = QB_LDS - QB_EXTRA;
@@ -477,55 +497,72 @@ __device__ inline void qb_build_t(const double* V, int rows, int nb, const doubl
 }
 
 // A (rows p0.., columns [c_begin, c_end) of X) -= V op(T) V^T A with
-// op(T) = T^T when trans (applying H^T) and T otherwise (applying H).
-__device__ inline void qb_apply(const double* V, int rows, int nb, const double* Tm, double* W, double* W2, double* X,
-                                int mx, int p0, int c_begin, int c_end, bool trans) {
-    const int tid = threadIdx.x, cc = tid / 16, rl = tid % 16;
-    for (int c0 = c_begin; c0 < c_end; c0 += QB_CW) {
-        const int cw = min(QB_CW, c_end - c0);
-        double acc[QB_NB];
+// op(T) = T^T when trans (applying H^T) and T otherwise (applying H), in
+// chunks of QB_WC columns on the fp64 MFMA (16x16x4): W = V^T A with rows as
+// the reduction dimension (one wave per 16-column tile), op(T) W per column,
+// then A -= V W computed as (W^T V^T) tiles of 16 columns x 16 rows so that
+// the stores run down the columns.
+__device__ inline void qb_apply(const double* V, int rows, int nb, const double* Tm, double* W, double* X, int mx,
+                                int p0, int c_begin, int c_end, bool trans) {
+    const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    constexpr int NW = QR_WG / WAVE;
+    for (int cb = c_begin; cb < c_end; cb += QB_WC) {
+        const int ncol = min(QB_WC, c_end - cb), ntile = (ncol + 15) / 16;
+        for (int t = wave; t < ntile; t += NW) {
+            const int c = t * 16 + lane % 16, j = lane % 16;
+            const double* xc = X + int64_t(cb + (c < ncol ? c : 0)) * mx + p0;
+            const double* vj = V + int64_t(j < nb ? j : 0) * rows;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            // 16 rows per iteration: the four global loads are issued
+            // before the four MFMAs that consume them
+            for (int r0 = 0; r0 < rows; r0 += 16) {
+                double av[4], bv[4];
 #pragma unroll
-        for (int j = 0; j < QB_NB; j++) acc[j] = 0.0;
-        if (cc < cw) {
-            const double* a = X + int64_t(c0 + cc) * mx + p0;
-            for (int r = rl; r < rows; r += 16) {
-                const double av = a[r];
+                for (int u = 0; u < 4; u++) {
+                    const int r = r0 + 4 * u + lane / 16;
+                    av[u] = (j < nb && r < rows) ? vj[r] : 0.0;
+                    bv[u] = (c < ncol && r < rows) ? xc[r] : 0.0;
+                }
 #pragma unroll
-                for (int j = 0; j < QB_NB; j++)
-                    if (j < nb) acc[j] += V[int64_t(j) * rows + r] * av;
+                for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int jj = lane / 16 + 4 * i;
+                if (jj < nb && c < ncol) W[jj * QB_WC + c] = acc[i];
             }
         }
-#pragma unroll
-        for (int j = 0; j < QB_NB; j++) {
-            double v = acc[j];
-            v += __shfl_xor(v, 1, WAVE);
-            v += __shfl_xor(v, 2, WAVE);
-            v += __shfl_xor(v, 4, WAVE);
-            v += __shfl_xor(v, 8, WAVE);
-            acc[j] = v;
-        }
-        if (rl == 0 && cc < cw)
-            for (int j = 0; j < nb; j++) W[j * QB_CW + cc] = acc[j];
         __syncthreads();
-        {
-            const int j = tid % 16, c = tid / 16;
-            if (j < nb && c < cw) {
+        for (int c = tid; c < ncol; c += QR_WG) {
+            double w[QB_NB];
+#pragma unroll
+            for (int j = 0; j < QB_NB; j++) w[j] = j < nb ? W[j * QB_WC + c] : 0.0;
+            for (int j = 0; j < nb; j++) {
                 double sacc = 0.0;
                 if (trans) {
-                    for (int i = 0; i <= j; i++) sacc += Tm[i * QB_NB + j] * W[i * QB_CW + c];
+                    for (int i = 0; i <= j; i++) sacc += Tm[i * QB_NB + j] * w[i];
                 } else {
-                    for (int i = j; i < nb; i++) sacc += Tm[j * QB_NB + i] * W[i * QB_CW + c];
+                    for (int i = j; i < nb; i++) sacc += Tm[j * QB_NB + i] * w[i];
                 }
-                W2[j * QB_CW + c] = sacc;
+                W[j * QB_WC + c] = sacc;
             }
         }
         __syncthreads();
-        if (cc < cw) {
-            double* a = X + int64_t(c0 + cc) * mx + p0;
-            for (int r = rl; r < rows; r += 16) {
-                double sacc = 0.0;
-                for (int j = 0; j < nb; j++) sacc += V[int64_t(j) * rows + r] * W2[j * QB_CW + cc];
-                a[r] -= sacc;
+        const int nrt = (rows + 15) / 16;
+        for (int q = wave; q < nrt * ntile; q += NW) {
+            const int rt = q / ntile, ct = q % ntile;
+            const int ca = ct * 16 + lane % 16, rb = rt * 16 + lane % 16;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < nb; k0 += 4) {
+                const int jk = k0 + lane / 16;
+                const double a = (jk < nb && ca < ncol) ? W[jk * QB_WC + ca] : 0.0;
+                const double b = (jk < nb && rb < rows) ? V[int64_t(jk) * rows + rb] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int c = ct * 16 + lane / 16 + 4 * i, r = rt * 16 + lane % 16;
+                if (c < ncol && r < rows) X[int64_t(cb + c) * mx + p0 + r] -= acc[i];
             }
         }
         __syncthreads();
@@ -545,8 +582,7 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
     double* Tm = sm;
     double* Gm = Tm + QB_NB * QB_NB;
     double* W = Gm + QB_NB * QB_NB;
-    double* W2 = W + QB_NB * QB_CW;
-    double* taus = W2 + QB_NB * QB_CW;
+    double* taus = W + QB_NB * QB_WC;
     double* V = taus + QB_NB;
     double* X = tk.work;
     double* tau = X + int64_t(mx) * nx;
@@ -617,7 +653,7 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
         __syncthreads();
         if (p0 + nb < nx) {
             qb_build_t(V, rows, nb, taus, Tm, Gm);
-            qb_apply(V, rows, nb, Tm, W, W2, X, mx, p0, p0 + nb, nx, true);
+            qb_apply(V, rows, nb, Tm, W, X, mx, p0, p0 + nb, nx, true);
         }
         __syncthreads();
     }
@@ -650,7 +686,7 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
         }
         __syncthreads();
         qb_build_t(V, rows, nb, taus, Tm, Gm);
-        qb_apply(V, rows, nb, Tm, W, W2, X, mx, p0, p0, k, false);
+        qb_apply(V, rows, nb, Tm, W, X, mx, p0, p0, k, false);
         __syncthreads();
     }
     if (!lq) {
@@ -1109,16 +1145,29 @@ void launch_permute_to_f64(const PermDev* tasks, const int64_t* prefix, const in
     check(hipGetLastError(), "permute launch");
 }
 
-void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream) {
+void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, int variant,
+                 hipStream_t stream) {
     if (num_tiles <= 0) return;
-    if (dtype == 0)
-        hipLaunchKernelGGL(gemm_kernel<double>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
-    else if (dtype == 1)
-        hipLaunchKernelGGL(gemm_kernel<float>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
-    else if (dtype == 2)
-        hipLaunchKernelGGL(gemm_kernel_bf16<2>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
+    const dim3 grid = dim3(static_cast<unsigned>(num_tiles));
+    if (dtype >= 2 && variant != 0) throw std::runtime_error("gemm: split-bf16 uses the 32x32 tile");
+    if (dtype == 0) {
+        if (variant == 0)
+            hipLaunchKernelGGL((gemm_kernel<double, 2, 2, 1, 1>), grid, dim3(256), 0, stream, tasks, tiles);
+        else if (variant == 1)
+            hipLaunchKernelGGL((gemm_kernel<double, 2, 2, 2, 2>), grid, dim3(256), 0, stream, tasks, tiles);
+        else
+            hipLaunchKernelGGL((gemm_kernel<double, 1, 1, 1, 1>), grid, dim3(64), 0, stream, tasks, tiles);
+    } else if (dtype == 1) {
+        if (variant == 0)
+            hipLaunchKernelGGL((gemm_kernel<float, 2, 2, 1, 1>), grid, dim3(256), 0, stream, tasks, tiles);
+        else if (variant == 1)
+            hipLaunchKernelGGL((gemm_kernel<float, 2, 2, 2, 2>), grid, dim3(256), 0, stream, tasks, tiles);
+        else
+            hipLaunchKernelGGL((gemm_kernel<float, 1, 1, 1, 1>), grid, dim3(64), 0, stream, tasks, tiles);
+    } else if (dtype == 2)
+        hipLaunchKernelGGL(gemm_kernel_bf16<2>, grid, dim3(WG), 0, stream, tasks, tiles);
     else if (dtype == 3)
-        hipLaunchKernelGGL(gemm_kernel_bf16<3>, dim3(unsigned(num_tiles)), dim3(WG), 0, stream, tasks, tiles);
+        hipLaunchKernelGGL(gemm_kernel_bf16<3>, grid, dim3(WG), 0, stream, tasks, tiles);
     else
         throw std::runtime_error("gemm: unsupported precision");
     check(hipGetLastError(), "gemm launch");

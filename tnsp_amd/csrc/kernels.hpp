@@ -65,13 +65,16 @@ void launch_permute_from_f64(const PermDev* tasks, const int64_t* prefix, const 
                              int dtype, hipStream_t stream);
 void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg, int dtype,
                  hipStream_t stream);
-constexpr int GEMM_TILE = 32;
 void launch_permute_to_f64(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg,
                            hipStream_t stream);
 // dtype: 0 float64, 1 float32, 2 float32 operands as split-bf16 x3 (hi*hi +
 // hi*lo + lo*hi), 3 split-bf16 x6 (three-term split, all products of order
 // <= 2); float32 accumulation for 1..3
-void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, hipStream_t stream);
+// variant: tile 0 = 32x32, 1 = 64x64, 2 = 16x16 (split-bf16: 32x32 only)
+void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, int dtype, int variant,
+                 hipStream_t stream);
+constexpr int GEMM_VARIANTS = 3;
+constexpr int GEMM_TILE_OF[GEMM_VARIANTS] = {32, 64, 16};
 // QR / SVD: one workgroup per block.  lds_bytes > 0 selects the LDS-resident
 // kernel with that much dynamic LDS (every task of the launch must fit);
 // 0 selects the kernel working in each task's global scratch.
@@ -97,7 +100,7 @@ inline int64_t svd_doubles(int64_t m, int64_t n) {
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream);
 constexpr int64_t QB_LDS = 156 * 1024;
 // rows for which the blocked kernel's panel keeps at least 4 columns
-constexpr int64_t QB_MAX_ROWS = (QB_LDS - 8 * (16 * 16 * 2 + 16 * 32 * 2 + 16)) / (8 * 4);
+constexpr int64_t QB_MAX_ROWS = (QB_LDS - 8 * (16 * 16 * 2 + 16 * 128 + 16)) / (8 * 4);
 // rpt > 0: register-resident kernel for blocks with at most 8 * rpt rows
 // (rpt in {8, 16, 24}), V recovered from U; otherwise LDS (lds_bytes > 0)
 // or global scratch.

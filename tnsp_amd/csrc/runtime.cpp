@@ -201,7 +201,10 @@ void Batch::flush_fill_as(int dt) {
 void Batch::flush_gemm_as(int prec) {
     const size_t before = cmds_.size();
     flush_gemm();
-    if (cmds_.size() > before) cmds_.back().dt = prec;
+    for (size_t i = before; i < cmds_.size(); i++) {
+        if (prec >= 2 && cmds_[i].rpt != 0) throw Error("split-bf16 GEMMs need the 32x32 tile for every class");
+        cmds_[i].dt = prec;
+    }
 }
 
 void Batch::flush_fill() {
@@ -225,32 +228,44 @@ void Batch::gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es)
         g.ta = t.ta;
         g.tb = t.tb;
         g.alpha = t.alpha;
-        pend_units_[GEMM] += 2.0 * double(t.m) * double(t.n) * double(t.k);
+        const double fl = 2.0 * double(t.m) * double(t.n) * double(t.k);
+        pend_units_[GEMM] += fl;
         const int32_t id = int32_t(pgemm_.size());
         pgemm_.push_back(g);
-        const int tm = (t.m + GEMM_TILE - 1) / GEMM_TILE, tn = (t.n + GEMM_TILE - 1) / GEMM_TILE;
+        const int v = gemm_table_[gemm_class(t.m, t.n)];
+        const int T = GEMM_TILE_OF[v];
+        pgemm_units_[v] += fl;
+        const int tm = (t.m + T - 1) / T, tn = (t.n + T - 1) / T;
         for (int i = 0; i < tm; i++)
-            for (int j = 0; j < tn; j++) ptile_.push_back(TileDev{id, i, j, 0});
+            for (int j = 0; j < tn; j++) ptile_[v].push_back(TileDev{id, i, j, 0});
     }
 }
 
 void Batch::flush_gemm() {
-    if (ptile_.empty()) {
+    bool any = false;
+    for (auto& pt : ptile_) any = any || !pt.empty();
+    if (!any) {
         pgemm_.clear();
         return;
     }
-    Cmd c;
-    c.kind = GEMM;
-    c.tasks = put(pgemm_);
-    c.prefix = put(ptile_);
-    c.wg = 0;
-    c.count = int64_t(pgemm_.size());
-    c.nwg = int64_t(ptile_.size());
-    c.units = pend_units_[GEMM];
+    const size_t tasks = put(pgemm_);
+    for (int v = 0; v < GEMM_VARIANTS; v++) {
+        if (ptile_[v].empty()) continue;
+        Cmd c;
+        c.kind = GEMM;
+        c.tasks = tasks;
+        c.prefix = put(ptile_[v]);
+        c.wg = 0;
+        c.count = int64_t(pgemm_.size());
+        c.nwg = int64_t(ptile_[v].size());
+        c.units = pgemm_units_[v];
+        c.rpt = v;
+        cmds_.push_back(c);
+        ptile_[v].clear();
+        pgemm_units_[v] = 0;
+    }
     pend_units_[GEMM] = 0;
-    cmds_.push_back(c);
     pgemm_.clear();
-    ptile_.clear();
 }
 
 void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, double* scratch) {
@@ -390,7 +405,8 @@ void Batch::run(hipStream_t st) {
                 break;
             case GEMM:
                 launch_gemm(reinterpret_cast<const GemmDev*>(base + c.tasks),
-                            reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, c.dt >= 0 ? c.dt : dtype_, st);
+                            reinterpret_cast<const TileDev*>(base + c.prefix), c.nwg, c.dt >= 0 ? c.dt : dtype_,
+                            c.rpt, st);
                 break;
             case QR:
                 launch_qr(reinterpret_cast<const QrDev*>(base + c.tasks), c.count, c.lds, c.rpt, dtype_, st);
@@ -419,7 +435,8 @@ void Batch::reset() {
     pdiag_.clear();
     pfill_.clear();
     pgemm_.clear();
-    ptile_.clear();
+    for (auto& pt : ptile_) pt.clear();
+    for (auto& u : pgemm_units_) u = 0;
     pqr_.clear();
     psvd_.clear();
     pqr_cost_.clear();

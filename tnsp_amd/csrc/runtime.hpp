@@ -77,6 +77,15 @@ class Batch {
     void flush_fill();
     void gemm(const std::vector<GemmTask>& ts, void* const* bufs, int64_t es);
     void flush_gemm();
+    // GEMM tile variant per sector size class (max(m, n) <= 16, <= 48,
+    // larger); set by the engine's autotuner
+    void set_gemm_table(const int* table) {
+        for (int c = 0; c < 3; c++) gemm_table_[c] = table[c];
+    }
+    static int gemm_class(int64_t m, int64_t n) {
+        const int64_t x = m > n ? m : n;
+        return x <= 16 ? 0 : (x <= 48 ? 1 : 2);
+    }
     // reduced-precision contraction pieces (float64 storage, float32 GEMM
     // operands): converting permutes, float32 fills and a GEMM launch in
     // precision `prec` (1 fp32, 2 split-bf16 x3, 3 split-bf16 x6)
@@ -133,7 +142,9 @@ class Batch {
     std::vector<PermDev> pperm_, pdiag_;
     std::vector<FillDev> pfill_;
     std::vector<GemmDev> pgemm_;
-    std::vector<TileDev> ptile_;
+    std::vector<TileDev> ptile_[GEMM_VARIANTS];
+    double pgemm_units_[GEMM_VARIANTS] = {0, 0, 0};
+    int gemm_table_[3] = {2, 0, 1};
     std::vector<QrDev> pqr_;
     std::vector<SvdDev> psvd_;
     std::vector<double> pqr_cost_, psvd_cost_;

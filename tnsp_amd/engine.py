@@ -60,6 +60,18 @@ class AmplitudeEngine:
     def set_chunk(self, chunk):
         _lib.check(_lib.lib().tns_engine_set_chunk(self.handle, int(chunk)))
 
+    def autotune(self, configs, cut=None):
+        """Pick the GEMM tile variant per sector size class by timing every
+        variant on the first contraction of ``configs`` (see
+        ``tns_engine_autotune``); returns (table, ms per class and variant)."""
+        cfg = np.ascontiguousarray(np.asarray(configs), dtype=np.int32)
+        remain, relative = cut_params(self.spec.cut if cut is None else cut)
+        table = (ctypes.c_int32 * 3)()
+        ms = (ctypes.c_double * 9)()
+        _lib.check(_lib.lib().tns_engine_autotune(self.handle, cfg.shape[0], cfg.ctypes.data_as(_lib.c_i32p), remain,
+                                                  relative, table, ms, ctypes.c_void_p(_stream())))
+        return list(table), np.array(list(ms)).reshape(3, 3)
+
     def evaluate(self, configs, cut=None):
         """psi for configurations of shape (n, L1, L2) (physical indices)."""
         cfg = np.ascontiguousarray(np.asarray(configs), dtype=np.int32)
