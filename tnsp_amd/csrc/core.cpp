@@ -71,6 +71,28 @@ void squash_into(PermTask& t, int rank, const int64_t* dims, const int64_t* sst,
     }
 }
 
+// a non-owning view of an edge's segments with its own arrow flag
+struct EdgeView {
+    const Segment* s = nullptr;
+    int n = 0;
+    bool arrow = false;
+    EdgeView() = default;
+    explicit EdgeView(const Edge& e) : s(e.segs.data()), n(int(e.segs.size())), arrow(e.arrow) {}
+    int size() const { return n; }
+    const Segment& seg(int64_t i) const { return s[i]; }
+    int find(const Sym& q) const {
+        for (int i = 0; i < n; i++)
+            if (s[i].q == q) return i;
+        return -1;
+    }
+    Edge to_edge() const {
+        Edge e;
+        e.segs.assign(s, s + n);
+        e.arrow = arrow;
+        return e;
+    }
+};
+
 struct GroupTable {
     // over sub-edge position tuples (row-major): target position, offset, odd count
     std::vector<int64_t> lead;
@@ -140,8 +162,13 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     const bool fermi = k.is_fermi;
     const int rank_bs = src.rank();
 
+    // edges are viewed (segments + arrow), not copied, until the result
+    // shape is built
     // step 1: cut
-    std::vector<Edge> edges_bs = src.edges;
+    std::vector<Edge> cut_store;
+    cut_store.reserve(args.cut.size());
+    std::vector<EdgeView> edges_bs(rank_bs);
+    for (int i = 0; i < rank_bs; i++) edges_bs[i] = EdgeView(src.edges[i]);
     bool has_cut = false;
     for (auto& c : args.cut) {
         int i = src.index_of(c.name);
@@ -149,24 +176,26 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
         has_cut = true;
         Edge ne;
         ne.arrow = edges_bs[i].arrow;
-        for (auto& s : edges_bs[i].segs) {
+        for (int si = 0; si < edges_bs[i].size(); si++) {
+            const Segment& sg = edges_bs[i].seg(si);
             bool found = false;
             for (auto& [q, nd] : c.dims) {
-                if (q == s.q) {
+                if (q == sg.q) {
                     found = true;
-                    if (nd != 0) ne.segs.push_back({s.q, std::min(nd, s.d)});
+                    if (nd != 0) ne.segs.push_back({sg.q, std::min(nd, sg.d)});
                     break;
                 }
             }
-            if (!found) ne.segs.push_back(s);
+            if (!found) ne.segs.push_back(sg);
         }
-        edges_bs[i] = std::move(ne);
+        cut_store.push_back(std::move(ne));
+        edges_bs[i] = EdgeView(cut_store.back());
     }
 
     // step 2: split
     std::vector<int32_t> names_as;
     std::vector<int> split_flags;
-    std::vector<Edge> edges_as;
+    std::vector<EdgeView> edges_as;
     std::vector<GroupTable> split_tab(rank_bs);
     std::vector<bool> is_split(rank_bs, false);
     for (int i = 0; i < rank_bs; i++) {
@@ -184,9 +213,9 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
         for (auto& [nm, e] : sp->parts) {
             names_as.push_back(nm);
             split_flags.push_back(i);
-            Edge ne = e;
+            EdgeView ne(e);
             ne.arrow = edges_bs[i].arrow;
-            edges_as.push_back(std::move(ne));
+            edges_as.push_back(ne);
         }
         const int nsub = int(sp->parts.size());
         GroupTable& g = split_tab[i];
@@ -194,19 +223,19 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
         int64_t total = 1;
         for (int s = nsub - 1; s >= 0; s--) {
             g.lead[s] = total;
-            total *= int64_t(edges_as[first + s].segs.size());
+            total *= int64_t(edges_as[first + s].size());
         }
         g.pos.assign(total, -1);
         g.off.assign(total, 0);
         g.npar.assign(total, 0);
-        std::vector<int64_t> bank(edges_bs[i].segs.size(), 0);
+        std::vector<int64_t> bank(edges_bs[i].size(), 0);
         std::vector<int64_t> idx(nsub, 0);
         for (int64_t lin = 0; lin < total; lin++) {
             Sym tot;
             int64_t sz = 1;
             int np = 0;
             for (int s = 0; s < nsub; s++) {
-                const Segment& seg = edges_as[first + s].segs[idx[s]];
+                const Segment& seg = edges_as[first + s].seg(idx[s]);
                 tot = sym_add(k, tot, seg.q);
                 sz *= seg.d;
                 np += sym_parity(k, seg.q);
@@ -219,7 +248,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
                 bank[p] += sz;
             }
             for (int s = nsub - 1; s >= 0; s--) {
-                if (++idx[s] < int64_t(edges_as[first + s].segs.size())) break;
+                if (++idx[s] < int64_t(edges_as[first + s].size())) break;
                 idx[s] = 0;
             }
         }
@@ -228,7 +257,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
 
     // step 3: reverse before transpose
     std::vector<char> rev_before(rank_t, 0);
-    std::vector<Edge> edges_bt = edges_as;
+    std::vector<EdgeView> edges_bt = edges_as;
     if (fermi) {
         for (int i = 0; i < rank_t; i++) {
             if (contains(args.reversed, names_as[i])) {
@@ -270,7 +299,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     }
 
     // step 5: reverse before merge, merge tables, result edges
-    std::vector<Edge> edges_bm(rank_t);
+    std::vector<EdgeView> edges_bm(rank_t);
     for (int i = 0; i < rank_t; i++) edges_bm[i] = edges_bt[d2s[i]];
     std::vector<char> rev_after(rank_t, 0);
     const bool any_merge = !args.merge.empty();
@@ -305,7 +334,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
                 int64_t total = 1;
                 for (int s = nsub - 1; s >= 0; s--) {
                     g.lead[s] = total;
-                    total *= int64_t(edges_bm[start + s].segs.size());
+                    total *= int64_t(edges_bm[start + s].size());
                 }
                 g.pos.assign(total, -1);
                 g.off.assign(total, 0);
@@ -318,7 +347,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
                     int64_t sz = 1;
                     int np = 0;
                     for (int s = 0; s < nsub; s++) {
-                        const Segment& seg = edges_bm[start + s].segs[idx[s]];
+                        const Segment& seg = edges_bm[start + s].seg(idx[s]);
                         tot = sym_add(k, tot, seg.q);
                         sz *= seg.d;
                         np += sym_parity(k, seg.q);
@@ -333,14 +362,14 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
                     g.npar[lin] = np;
                     merged.segs[p].d += sz;
                     for (int s = nsub - 1; s >= 0; s--) {
-                        if (++idx[s] < int64_t(edges_bm[start + s].segs.size())) break;
+                        if (++idx[s] < int64_t(edges_bm[start + s].size())) break;
                         idx[s] = 0;
                     }
                 }
                 out.edges[j] = std::move(merged);
             } else {
                 if (end == start) throw Error("internal: empty group without merge");
-                out.edges[j] = edges_bm[start];
+                out.edges[j] = edges_bm[start].to_edge();
             }
             start = end;
         }
@@ -363,10 +392,10 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     Sym const_sum;
     bool const_par = false;
     for (int i = 0; i < rank_t; i++) {
-        if (edges_bm[i].segs.size() == 0) return;  // no blocks at all
-        if (edges_bm[i].segs.size() == 1) {
-            const_sum = sym_add(k, const_sum, edges_bm[i].segs[0].q);
-            if (fermi && rev_flag[i] && sym_parity(k, edges_bm[i].segs[0].q)) const_par = !const_par;
+        if (edges_bm[i].size() == 0) return;  // no blocks at all
+        if (edges_bm[i].size() == 1) {
+            const_sum = sym_add(k, const_sum, edges_bm[i].seg(0).q);
+            if (fermi && rev_flag[i] && sym_parity(k, edges_bm[i].seg(0).q)) const_par = !const_par;
         } else {
             active.push_back(i);
         }
@@ -379,20 +408,34 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     odd_list.reserve(rank_t);
     const int na = int(active.size());
     std::vector<int64_t> aidx(na, 0);
+    // the odometer runs over all active edges but the last; the last edge's
+    // segment is the one whose charge closes the total (charges are unique
+    // within an edge), so forbidden tuples are never visited
+    const int nfree = na > 0 ? na - 1 : 0;
     while (true) {
         Sym tot = const_sum;
-        for (int a = 0; a < na; a++) tot = sym_add(k, tot, edges_bm[active[a]].segs[aidx[a]].q);
-        if (sym_is_zero(tot)) {
+        for (int a = 0; a < nfree; a++) tot = sym_add(k, tot, edges_bm[active[a]].seg(aidx[a]).q);
+        bool allowed = true;
+        if (na > 0) {
+            const int pl = edges_bm[active[na - 1]].find(sym_neg(k, tot));
+            if (pl < 0) {
+                allowed = false;
+            } else {
+                aidx[na - 1] = pl;
+                tot = sym_add(k, tot, edges_bm[active[na - 1]].seg(pl).q);
+            }
+        }
+        if (allowed && sym_is_zero(tot)) {
             for (int a = 0; a < na; a++) pos_bm[active[a]] = aidx[a];
             bool par = const_par;
             if (fermi) {
                 for (int a = 0; a < na; a++) {
                     int i = active[a];
-                    if (rev_flag[i] && sym_parity(k, edges_bm[i].segs[pos_bm[i]].q)) par = !par;
+                    if (rev_flag[i] && sym_parity(k, edges_bm[i].seg(pos_bm[i]).q)) par = !par;
                 }
                 odd_list.clear();
                 for (int i = 0; i < rank_t; i++)
-                    if (sym_parity(k, edges_bm[i].segs[pos_bm[i]].q)) odd_list.push_back(i);
+                    if (sym_parity(k, edges_bm[i].seg(pos_bm[i]).q)) odd_list.push_back(i);
                 for (size_t x = 0; x < odd_list.size(); x++)
                     for (size_t y = x + 1; y < odd_list.size(); y++)
                         if (d2s[odd_list[x]] > d2s[odd_list[y]]) par = !par;
@@ -401,7 +444,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
             for (int i = 0; i < rank_t; i++) {
                 const int64_t p = pos_bm[i];
                 pos_as[d2s[i]] = p;
-                const int64_t d = edges_bm[i].segs[p].d;
+                const int64_t d = edges_bm[i].seg(p).d;
                 dims_bm[i] = d;
                 dims_as[d2s[i]] = d;
                 if (d == 0) empty = true;
@@ -445,7 +488,7 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
             }
             if (!skip && !empty) {
                 if (has_cut) {
-                    for (int j = 0; j < rank_bs; j++) pos_src[j] = src.edges[j].find(edges_bs[j].segs[pos_bs[j]].q);
+                    for (int j = 0; j < rank_bs; j++) pos_src[j] = src.edges[j].find(edges_bs[j].seg(pos_bs[j]).q);
                 } else {
                     for (int j = 0; j < rank_bs; j++) pos_src[j] = pos_bs[j];
                 }
@@ -479,10 +522,10 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
                 tasks.push_back(t);
             }
         }
-        // advance odometer over active edges
-        int a = na - 1;
+        // advance the odometer over the free active edges
+        int a = nfree - 1;
         while (a >= 0) {
-            if (++aidx[a] < int64_t(edges_bm[active[a]].segs.size())) break;
+            if (++aidx[a] < int64_t(edges_bm[active[a]].size())) break;
             aidx[a] = 0;
             a--;
         }

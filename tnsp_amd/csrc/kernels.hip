@@ -907,12 +907,24 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
 // (lane = 8 * row_group + slot % 8),
 // each keeping RPT rows of the slot's two columns in registers.  After every
 // round the columns move along the round-robin ring (slot 0's top fixed,
-// tops shift up, bottoms shift down) through LDS in chunks of rows.
+// tops shift up, bottoms shift down): by DPP within a wave, through LDS
+// across waves.
 // V is not accumulated: after convergence it is recovered as
 // V = M^T U diag(1/sigma) (an LDS-tiled product), which is exact up to
 // eps * sigma_max / sigma_j; columns with sigma_j <= 1e-13 * sigma_max get
 // V = 0 (their products U S V^T are below that level).  The engine only
 // consumes U S and V^T of kept values, so no orthonormal completion is done.
+
+constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111;
+
+// a double moved across lanes by a DPP row shift (two 32-bit halves)
+template<int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
+}
 
 __device__ inline double sum8(double v) {
     v += __shfl_xor(v, 8, WAVE);
@@ -923,11 +935,11 @@ __device__ inline double sum8(double v) {
 
 template<typename T, int RPT>
 __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restrict__ tasks) {
-    // rows exchanged per LDS pass: smaller for the largest tile, which keeps
-    // the kernel within the 168 registers of three waves per SIMD
-    constexpr int SVR_NT = 32 * RPT, SVR_SLOTS = 4 * RPT, SVR_CH = RPT >= 24 ? 4 : 8;
-    extern __shared__ double xbuf[];  // 2 x SVR_CH x SVR_SLOTS x 8 doubles
-    __shared__ int ids[2][SVR_SLOTS];
+    constexpr int SVR_NT = 32 * RPT, SVR_SLOTS = 4 * RPT, NWV = SVR_NT / WAVE;
+    // wave-boundary columns: 2 parities x (top, bottom) x NWV x 8 x RPT
+    // doubles; reused for the recovery tiles
+    extern __shared__ double xbuf[];
+    __shared__ int ibuf[2][2][NWV];
     __shared__ double smax;
     const SvdDev tk = tasks[blockIdx.x];
     const int m = tk.m, n = tk.n;
@@ -940,7 +952,7 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
         for (int c = tid; c < kx; c += SVR_NT) tk.s[c] = __builtin_nan("");
         return;
     }
-    const int gl = lane / 8, p = w * 8 + lane % 8;
+    const int gl = lane / 8, pl = lane % 8, p = w * 8 + pl;
     const int h = (kx + 1) / 2;
     const bool act = p < h;
     int ca = p, cb = p + h;
@@ -982,36 +994,45 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
             }
             if (h < 2) continue;
             // ring step: top[p] <- top[p-1] (top[1] <- bot[0], top[0] stays),
-            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1])
+            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1]).  Slots p -+ 1 of the
+            // same wave sit in the neighbouring lane of the same 16-lane row
+            // (DPP row shift); slots across a wave boundary go through LDS,
+            // double-buffered by round parity so one barrier per round
+            // suffices.
+            {
+                const int par = round & 1;
+                double* btop = xbuf + par * (2 * NWV * 8 * RPT);
+                double* bbot = btop + NWV * 8 * RPT;
+                int* itop = ibuf[par][0];
+                int* ibot = ibuf[par][1];
+                if (pl == 7) {
 #pragma unroll
-            for (int c0 = 0; c0 < RPT; c0 += SVR_CH) {
-                if (act) {
+                    for (int i = 0; i < RPT; i++) btop[(w * 8 + gl) * RPT + i] = xa[i];
+                    if (gl == 0) itop[w] = ca;
+                }
+                if (pl == 0) {
 #pragma unroll
-                    for (int i = 0; i < SVR_CH; i++) {
-                        xbuf[((0 * SVR_CH + i) * SVR_SLOTS + p) * 8 + gl] = p == 0 ? xb[c0 + i] : xa[c0 + i];
-                        xbuf[((1 * SVR_CH + i) * SVR_SLOTS + p) * 8 + gl] = xb[c0 + i];
-                    }
-                    if (c0 == 0 && gl == 0) {
-                        ids[0][p] = p == 0 ? cb : ca;
-                        ids[1][p] = cb;
-                    }
+                    for (int i = 0; i < RPT; i++) bbot[(w * 8 + gl) * RPT + i] = xb[i];
+                    if (gl == 0) ibot[w] = cb;
                 }
                 __syncthreads();
-                if (act) {
 #pragma unroll
-                    for (int i = 0; i < SVR_CH; i++) {
-                        const double told = xa[c0 + i];
-                        if (p >= 1) xa[c0 + i] = xbuf[((0 * SVR_CH + i) * SVR_SLOTS + p - 1) * 8 + gl];
-                        xb[c0 + i] = p <= h - 2 ? xbuf[((1 * SVR_CH + i) * SVR_SLOTS + p + 1) * 8 + gl] : told;
-                    }
-                    if (c0 == 0) {
-                        const int nca = p >= 1 ? ids[0][p - 1] : ca;
-                        const int ncb = p <= h - 2 ? ids[1][p + 1] : ca;
-                        ca = nca;
-                        cb = ncb;
-                    }
+                for (int i = 0; i < RPT; i++) {
+                    double t = dpp_f64<DPP_ROW_SHR1>(p == 0 ? xb[i] : xa[i]);
+                    double b = dpp_f64<DPP_ROW_SHL1>(xb[i]);
+                    if (pl == 0) t = w > 0 ? btop[((w - 1) * 8 + gl) * RPT + i] : xa[i];
+                    if (pl == 7) b = w + 1 < NWV ? bbot[((w + 1) * 8 + gl) * RPT + i] : 0.0;
+                    const double nb = p == h - 1 ? xa[i] : b;
+                    if (p != 0) xa[i] = t;
+                    xb[i] = nb;
                 }
-                __syncthreads();
+                int ti = __builtin_amdgcn_update_dpp(0, p == 0 ? cb : ca, DPP_ROW_SHR1, 0xF, 0xF, true);
+                int bi = __builtin_amdgcn_update_dpp(0, cb, DPP_ROW_SHL1, 0xF, 0xF, true);
+                if (pl == 0) ti = w > 0 ? itop[w - 1] : ca;
+                if (pl == 7) bi = w + 1 < NWV ? ibot[w + 1] : 0;
+                const int ncb = p == h - 1 ? ca : bi;
+                if (p != 0) ca = ti;
+                cb = ncb;
             }
         }
         if (!__syncthreads_or(rot)) break;
@@ -1216,10 +1237,9 @@ void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, 
 namespace {
 template<typename T, int RPT>
 void launch_svd_reg(const SvdDev* tasks, int64_t num, hipStream_t stream) {
-    // exchange buffer (2 x 8 rows x slots x 8 lanes), at least the 32 KiB of
+    // wave-boundary buffer (2 x 2 x waves x 8 x RPT), at least the 32 KiB of
     // the recovery tiles
-    const size_t ch = RPT >= 24 ? 4 : 8;
-    const size_t lds = std::max<size_t>(2 * ch * 4 * RPT * 8, 2 * 32 * 64) * sizeof(double);
+    const size_t lds = std::max<size_t>(size_t(2 * 2 * (RPT / 2) * 8 * RPT), 2 * 32 * 64) * sizeof(double);
     allow_lds(svd_kernel_reg<T, RPT>, int64_t(lds));
     hipLaunchKernelGGL((svd_kernel_reg<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
 }
