@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--ess-samples", type=int, default=16)
     ap.add_argument("--ess-train-sweeps", type=int, default=6)
     ap.add_argument("--ess-epochs", type=int, default=6)
+    ap.add_argument("--ess-learned-k", default="",
+                    help="proposals per learned sample, comma-separated (default: bonds / 2)")
     return ap.parse_args()
 
 
@@ -153,16 +155,22 @@ def ess_comparison(args, log):
     t0 = time.time()
     losses = train(net, occs, hops, targets, epochs=args.ess_epochs, batch=1024, lr=2e-3, log=log)
     t_train = time.time() - t0
-    res_plain = measure_ess(plain.sweep, plain.local_energy, args.ess_samples, sites=L)
+    res_plain = measure_ess(plain.sweep, plain.local_energy, args.ess_samples, sites=L,
+                            log=lambda m: log(f"plain sweep: {m}"))
     res_plain["acceptance"] = plain.accepted / max(1, plain.proposals)
-    learned = LearnedSampler(eng, net, plain.configs.copy(), seed=2)
-    K = max(1, len(plain.bonds) // 2)
-    res_learned = measure_ess(lambda: [learned.step() for _ in range(K)],
-                              lambda: plain.local_energy(learned.configs, learned.psi), args.ess_samples, sites=L)
-    res_learned["acceptance"] = learned.accepted / max(1, learned.proposals)
-    res_learned["proposals_per_sample"] = K
+    ks = [int(x) for x in args.ess_learned_k.split(",") if x] or [max(1, len(plain.bonds) // 2)]
+    res_learned = []
+    for K in ks:
+        learned = LearnedSampler(eng, net, plain.configs.copy(), seed=2)
+        r = measure_ess(lambda: [learned.step() for _ in range(K)],
+                        lambda: plain.local_energy(learned.configs, learned.psi), args.ess_samples, sites=L,
+                        log=lambda m: log(f"learned K={K}: {m}"))
+        r["acceptance"] = learned.accepted / max(1, learned.proposals)
+        r["proposals_per_sample"] = K
+        res_learned.append(r)
+    best = max(res_learned, key=lambda r: r["ess_per_s"])
     return {"workload": args.ess, "chains": len(start), "plain": res_plain, "learned": res_learned,
-            "gain": res_learned["ess_per_s"] / max(res_plain["ess_per_s"], 1e-12),
+            "gain": best["ess_per_s"] / max(res_plain["ess_per_s"], 1e-12),
             "training": {"samples": int(len(hops)), "collect_s": t_collect, "train_s": t_train,
                          "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}}
 

@@ -183,7 +183,7 @@ def integrated_autocorrelation(series, c=5.0):
     return max(tau, 1e-9)
 
 
-def measure_ess(advance, energy, samples, burn_in=2, sites=1):
+def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
     """Calls ``advance()`` ``burn_in`` + ``samples`` times, recording
     ``energy()`` (per-chain local energies) after each timed advance; returns
     ESS of the energy estimator, the sampling seconds (energy evaluation not
@@ -197,6 +197,8 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1):
         advance()
         t_sample += time.time() - t0
         energies.append(energy())
+        if log and (len(energies) % 8 == 0 or len(energies) == samples):
+            log(f"{len(energies)}/{samples} samples, {t_sample:.1f}s sampling")
     series = np.stack(energies, axis=1)
     tau = integrated_autocorrelation(series)
     ess = series.size / tau
