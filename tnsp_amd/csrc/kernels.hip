@@ -915,7 +915,7 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
 // V = 0 (their products U S V^T are below that level).  The engine only
 // consumes U S and V^T of kept values, so no orthonormal completion is done.
 
-constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111;
+constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111, DPP_ROW_ROR8 = 0x128;
 
 // a double moved across lanes by a DPP row shift (two 32-bit halves)
 template<int CTRL>
@@ -926,8 +926,11 @@ __device__ inline double dpp_f64(double v) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
 }
 
+// butterfly sum over the 8 lanes {l ^ 8k}: the first level by a DPP rotate
+// by 8 within the 16-lane row (lane l ^ 8), the others through LDS shuffles;
+// every lane ends with the same value (the additions commute)
 __device__ inline double sum8(double v) {
-    v += __shfl_xor(v, 8, WAVE);
+    v += dpp_f64<DPP_ROW_ROR8>(v);
     v += __shfl_xor(v, 16, WAVE);
     v += __shfl_xor(v, 32, WAVE);
     return v;
@@ -968,13 +971,20 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
         int rot = 0;
         for (int round = 0; round < 2 * h - 1; round++) {
             if (act) {
-                double saa = 0, sbb = 0, sab = 0;
+                // two partial sums per product: halves the FMA dependency chains
+                double saa = 0, sbb = 0, sab = 0, saa1 = 0, sbb1 = 0, sab1 = 0;
 #pragma unroll
-                for (int i = 0; i < RPT; i++) {
+                for (int i = 0; i < RPT; i += 2) {
                     saa += xa[i] * xa[i];
                     sbb += xb[i] * xb[i];
                     sab += xa[i] * xb[i];
+                    saa1 += xa[i + 1] * xa[i + 1];
+                    sbb1 += xb[i + 1] * xb[i + 1];
+                    sab1 += xa[i + 1] * xb[i + 1];
                 }
+                saa += saa1;
+                sbb += sbb1;
+                sab += sab1;
                 saa = sum8(saa);
                 sbb = sum8(sbb);
                 sab = sum8(sab);
