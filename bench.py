@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--cpu-configs", type=int, default=2, help="configs per host process for the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain", type=int, default=0, help="also time N configs on the plain per-config path")
+    ap.add_argument("--plain", type=int, default=2, help="also time N configs on the plain per-config path")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="contraction GEMM precision of the timed engine")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile per size class")
@@ -231,11 +231,16 @@ def main():
         roof["traffic"] = measured_traffic(dom[0], args)
     plain = None
     if args.plain:
+        # the plain path: the drop-in Tensor API, one configuration and one
+        # op at a time (tetragono's own loop on the GPU); first call warms up
+        peps.amplitude_plain(lat, configs[0])
+        torch.cuda.synchronize()
         t1 = time.time()
-        for c in configs[:args.plain]:
+        for c in configs[1:1 + args.plain]:
             peps.amplitude_plain(lat, c)
         torch.cuda.synchronize()
         plain = args.plain / (time.time() - t1)
+        print(f"[bench] plain per-configuration path {plain:.2f} configs/s", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
@@ -270,7 +275,8 @@ def main():
                   "gemm_gflop": st["gemm_flops"] / 1e9, "qr_gflop": st["qr_flops"] / 1e9,
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
-                  "plain_path_configs_per_s": plain, "precision": args.precision, "gemm_tiles": tuned, "ess": ess},
+                  "plain_path_configs_per_s": plain,
+                  "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned, "ess": ess},
     }
     print(json.dumps(line), flush=True)
 

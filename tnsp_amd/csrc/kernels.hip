@@ -26,33 +26,58 @@ __device__ inline T neg_if(T v, int neg) {
 }
 
 // ---------------------------------------------------------------- permute
+// Each thread handles PERM_CHUNK / WG elements, WG apart: the offsets of all
+// of them are decoded first, then all loads are issued before the stores, so
+// a thread keeps that many gathers in flight instead of one.
 template<typename TS, typename T>
 __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__ tasks, const int64_t* __restrict__ prefix,
                                                      const int32_t* __restrict__ wg_first) {
+    constexpr int PER = PERM_CHUNK / WG;
     const int64_t base = int64_t(blockIdx.x) * PERM_CHUNK;
+    const int t_end = wg_first[gridDim.x];
     int t = wg_first[blockIdx.x];
-    for (int j = 0; j < PERM_CHUNK / WG; j++) {
+    const TS* src[PER];
+    T* dst[PER];
+    int64_t so[PER], dof[PER];
+    bool neg[PER], ok[PER];
+    int64_t end = prefix[t + 1];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
         const int64_t e = base + threadIdx.x + int64_t(j) * WG;
         // advance to the task containing e (tasks are contiguous in e)
-        int64_t end = prefix[t + 1];
-        while (end <= e) {
+        while (end <= e && t < t_end) {
             t++;
-            if (t >= wg_first[gridDim.x]) return;
-            end = prefix[t + 1];
+            if (t < t_end) end = prefix[t + 1];
         }
-        const PermDev& tk = tasks[t];
-        uint32_t local = uint32_t(e - prefix[t]);
-        int64_t so = 0, dof = 0;
-        for (int r = tk.rank - 1; r >= 0; r--) {
-            const uint32_t d = uint32_t(tk.dims[r]);
-            const uint32_t i = local % d;
-            local /= d;
-            so += int64_t(i) * tk.ss[r];
-            dof += int64_t(i) * tk.ds[r];
+        ok[j] = t < t_end;
+        so[j] = dof[j] = 0;
+        src[j] = nullptr;
+        dst[j] = nullptr;
+        neg[j] = false;
+        if (ok[j]) {
+            const PermDev& tk = tasks[t];
+            uint32_t local = uint32_t(e - prefix[t]);
+            int64_t a = 0, c = 0;
+            for (int r = tk.rank - 1; r >= 0; r--) {
+                const uint32_t d = uint32_t(tk.dims[r]);
+                const uint32_t i = local % d;
+                local /= d;
+                a += int64_t(i) * tk.ss[r];
+                c += int64_t(i) * tk.ds[r];
+            }
+            so[j] = a;
+            dof[j] = c;
+            src[j] = static_cast<const TS*>(tk.src);
+            dst[j] = static_cast<T*>(tk.dst);
+            neg[j] = tk.neg != 0;
         }
-        const T v = T(static_cast<const TS*>(tk.src)[so]);
-        static_cast<T*>(tk.dst)[dof] = neg_if(v, tk.neg);
     }
+    T v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) v[j] = ok[j] ? T(src[j][so[j]]) : T(0);
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+        if (ok[j]) dst[j][dof[j]] = neg_if(v[j], neg[j] ? 1 : 0);
 }
 
 template<typename T>
