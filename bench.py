@@ -149,7 +149,7 @@ def ess_comparison(args, log):
     start = cand[psi != 0][:args.ess_chains]
     plain = TVSampler(eng, start, seed=1)
     t0 = time.time()
-    occs, hops, targets = collect_training_data(plain, args.ess_train_sweeps)
+    occs, hops, targets = collect_training_data(plain, args.ess_train_sweeps, log=log)
     t_collect = time.time() - t0
     net = HopProposalNet(spec.L1, spec.L2, channels=32, layers=4, seed=3)
     t0 = time.time()
@@ -271,7 +271,8 @@ def main():
         "cpu_baseline": cpu,
         "extra": {"kernel_ms": {k: v["ms"] for k, v in ks.items()},
                   "kernel_launches": {k: v["launches"] for k, v in ks.items()},
-                  "host_plan_s": st["plan_s"], "engine_total_s": st["total_s"],
+                  "host_plan_s": st["plan_s"], "host_bind_s": st["bind_s"], "svals_wait_s": st["sync_s"],
+                  "engine_total_s": st["total_s"],
                   "gemm_gflop": st["gemm_flops"] / 1e9, "qr_gflop": st["qr_flops"] / 1e9,
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,

@@ -122,7 +122,9 @@ int tns_engine_autotune(tns_engine* e, int64_t nconf, const int32_t* occupations
 int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations, int64_t remain_cut,
                         double relative_cut, double* psi, void* stream);
 /* values: plan_s exec_s total_s gemm_flops qr_flops svd_flops perm_elems
- * launches device_bytes configs (n <= 10) */
+ * launches device_bytes configs bind_s sync_s (n <= 12); plan_s is the
+ * parallel per-configuration planning, bind_s the serial binding and task
+ * packing, sync_s the waits for singular values (summed over lanes) */
 int tns_engine_stats(const tns_engine* e, double* values, int n);
 int tns_engine_reset_stats(tns_engine* e);
 /* per-kernel device timing with HIP events on the launch stream: permute,

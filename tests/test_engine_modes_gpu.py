@@ -1,0 +1,65 @@
+"""Engine options on the GPU: GEMM precision modes and the tile autotuner.
+
+Precision: fp32 / split-bf16 contraction GEMMs with float64 storage, QR and
+SVD.  Bars: every mode agrees in sign with the float64 engine on the golden
+4x4 lattice, and |d log psi| stays below a bound per mode, set at ~30x the
+largest deviation measured on the 8x8 D=8 lattice (fp32 and bf16x6 ~1e-3,
+bf16x3 ~0.2).  The float64 engine itself is pinned to TAT in test_golden.py.
+Autotuner: the table is a valid variant per class and psi does not change.
+"""
+import numpy as np
+import pytest
+
+from helpers import load_golden
+
+pytestmark = pytest.mark.gpu
+
+_, _, AMPS = load_golden()
+BOUND = {"fp32": 3e-2, "bf16x6": 3e-2, "bf16x3": 1.0}
+
+
+def c2_case():
+    from tnsp_amd import peps as ppeps
+
+    case = AMPS["C2_4x4_D4_Dc16"]
+    sd = case["spec"]
+    spec = ppeps.LatticeSpec(sd["kind"], sd["L1"], sd["L2"], tuple(sd["segments"]), sd["N"], sd["cut"], sd["seed"])
+    return spec, ppeps.random_storages(spec), np.array(case["configs"]), np.array([p["value"] for p in case["psi"]])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6", "bf16x3"])
+def test_reduced_precision_gemms(precision):
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, ref = c2_case()
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages), chunk=16, lanes=2, precision=precision)
+    got = eng.evaluate(configs)
+    assert np.array_equal(np.sign(got), np.sign(ref))
+    d = np.abs(np.log(np.abs(got)) - np.log(np.abs(ref)))
+    assert d.max() < BOUND[precision], (precision, d.max())
+    # switching back gives the float64 result
+    eng.set_precision("fp64")
+    np.testing.assert_allclose(eng.evaluate(configs), ref, rtol=1e-10)
+
+
+def test_precision_rejected_on_float32_engine():
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, ref = c2_case()
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages, dtype="float32"), chunk=16)
+    with pytest.raises(RuntimeError):
+        eng.set_precision("bf16x3")
+
+
+def test_autotune_keeps_psi():
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, ref = c2_case()
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages), chunk=8, lanes=3)
+    table, ms = eng.autotune(configs[:8])
+    assert len(table) == 3 and all(v in (0, 1, 2) for v in table)
+    assert ms.shape == (3, 3) and np.all(ms >= 0)
+    np.testing.assert_allclose(eng.evaluate(configs), ref, rtol=1e-10)

@@ -281,7 +281,7 @@ void Engine::contract(std::vector<Slot*>& a, std::vector<Slot*>& b,
         bt.flush_perm_cvt(2);
     else
         bt.flush_perm();
-    stats_.plan_s += now() - t0;
+    stats_.bind_s += now() - t0;
     stats_.launches += 4;
     run_batch(bt, st);
     for (int64_t i = 0; i < B; i++) {
@@ -340,7 +340,7 @@ void Engine::qr(std::vector<Slot*>& t, const std::vector<int32_t>& free_names, i
         bt.perm(plans[i].perm_2, bufs, es_);
     }
     bt.flush_perm();
-    stats_.plan_s += now() - t0;
+    stats_.bind_s += now() - t0;
     stats_.launches += 3;
     run_batch(bt, st);
     for (int64_t i = 0; i < B; i++) {
@@ -398,14 +398,14 @@ void Engine::svd(std::vector<Slot*>& t, const std::vector<int32_t>& free_u, int3
         bt.svd(plans[i].svds, bufs, es_, reinterpret_cast<double*>(T + ts[i]));
     }
     bt.flush_svd();
-    stats_.plan_s += now() - t0;
+    stats_.bind_s += now() - t0;
     stats_.launches += 2;
     run_batch(bt, st);
     svals_host_.resize(sv_bytes / 8 + 1);
     const double t1s = now();
     if (sv_bytes) check_hip(hipMemcpyAsync(svals_host_.data(), S, sv_bytes, hipMemcpyDeviceToHost, st), "svals copy");
     check_hip(hipStreamSynchronize(st), "svals sync");
-    stats_.exec_s += now() - t1s;
+    stats_.sync_s += now() - t1s;
     if (profile_) timer_.collect();
     t0 = now();
     pool_.parallel_for(B, [&](int64_t b, int64_t e) {
@@ -439,7 +439,7 @@ void Engine::svd(std::vector<Slot*>& t, const std::vector<int32_t>& free_u, int3
         b2.diag(plans[i].diag_s, bufs, es_);
     }
     b2.flush_diag();
-    stats_.plan_s += now() - t0;
+    stats_.bind_s += now() - t0;
     stats_.launches += 3;
     run_batch(b2, st);
     for (int64_t i = 0; i < B; i++) {
@@ -839,18 +839,18 @@ int tns_engine_evaluate(tns_engine* e, int64_t nconf, const int32_t* occupations
 
 int tns_engine_stats(const tns_engine* e, double* values, int n) {
     TNS_TRY({
-        double v[11] = {0};
+        double v[12] = {0};
         for (auto& l : e->lanes) {
             const auto& s = l->stats();
-            const double x[11] = {s.plan_s,     s.exec_s,   s.total_s,    s.gemm_flops, s.qr_flops, s.svd_flops,
-                                  s.perm_elems, s.launches, s.peak_bytes, s.configs,    0.0};
-            for (int i = 0; i < 11; i++) v[i] += x[i];
+            const double x[12] = {s.plan_s,     s.exec_s,   s.total_s,    s.gemm_flops, s.qr_flops, s.svd_flops,
+                                  s.perm_elems, s.launches, s.peak_bytes, s.configs,    s.bind_s,   s.sync_s};
+            for (int i = 0; i < 12; i++) v[i] += x[i];
         }
         // wall-clock style entries: the lanes run concurrently
         double tmax = 0;
         for (auto& l : e->lanes) tmax = std::max(tmax, l->stats().total_s);
         v[2] = tmax;
-        for (int i = 0; i < n && i < 11; i++) values[i] = v[i];
+        for (int i = 0; i < n && i < 12; i++) values[i] = v[i];
     })
 }
 

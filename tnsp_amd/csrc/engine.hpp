@@ -64,7 +64,10 @@ struct Slot {
 };
 
 struct EngineStats {
-    double plan_s = 0, exec_s = 0, total_s = 0;
+    // plan_s: per-configuration planning on the thread pool; bind_s: the
+    // lane thread's serial arena binding and task packing; exec_s: upload and
+    // launch calls; sync_s: waiting for singular values before truncation
+    double plan_s = 0, exec_s = 0, total_s = 0, bind_s = 0, sync_s = 0;
     double gemm_flops = 0, qr_flops = 0, svd_flops = 0, perm_elems = 0;
     double launches = 0, peak_bytes = 0, configs = 0;
 };

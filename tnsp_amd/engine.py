@@ -18,7 +18,7 @@ from .shape import NAMES
 from .tensor import _stream
 
 STAT_NAMES = ["plan_s", "exec_s", "total_s", "gemm_flops", "qr_flops", "svd_flops", "perm_elems", "launches",
-              "device_bytes", "configs"]
+              "device_bytes", "configs", "bind_s", "sync_s"]
 
 
 class AmplitudeEngine:

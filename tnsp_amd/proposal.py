@@ -220,7 +220,7 @@ class LearnedSampler:
         return self.configs
 
 
-def collect_training_data(sampler, sweeps, max_samples=None):
+def collect_training_data(sampler, sweeps, max_samples=None, log=None):
     """Runs plain sweeps and records every tried hop: (configuration before,
     hop index, log|psi(s')/psi(s)|).  ``sampler`` is a TVSampler."""
     occs, hops, targets = [], [], []
@@ -238,6 +238,8 @@ def collect_training_data(sampler, sweeps, max_samples=None):
             hops.append(bond_hop(before, ia, ib, L2, a[0] == b[0]))
             targets.append(t)
             count += len(idx)
+        if log:
+            log(f"training data: sweep done, {count} hops")
         if max_samples and count >= max_samples:
             break
     return np.concatenate(occs), np.concatenate(hops).astype(np.int32), np.concatenate(targets).astype(np.float32)

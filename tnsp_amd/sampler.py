@@ -197,7 +197,7 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
         advance()
         t_sample += time.time() - t0
         energies.append(energy())
-        if log and (len(energies) % 8 == 0 or len(energies) == samples):
+        if log:
             log(f"{len(energies)}/{samples} samples, {t_sample:.1f}s sampling")
     series = np.stack(energies, axis=1)
     tau = integrated_autocorrelation(series)
