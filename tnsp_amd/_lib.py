@@ -49,6 +49,14 @@ _SIGNATURES = {
     "tns_plan_buffers": [c_vp, c_i64p, ctypes.POINTER(ctypes.c_int)],
     "tns_svd_sectors": [c_vp, c_i64p, c_i64p, c_i32p, ctypes.POINTER(ctypes.c_int)],
     "tns_svd_truncate": [c_vp, c_dblp, ctypes.c_int64, ctypes.c_double],
+    "tns_plan_contract_many": [ctypes.c_int, c_vpp, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int, c_vpp],
+    "tns_plan_qr_many": [ctypes.c_int, c_vpp, ctypes.c_char, ctypes.c_int, c_i32p, ctypes.c_int32, ctypes.c_int32,
+                         ctypes.c_int, c_vpp],
+    "tns_plan_svd_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int, c_vpp],
+    "tns_plans_run": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, c_vpp, c_vp],
+    "tns_svd_factor_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int64, ctypes.c_double,
+                            ctypes.c_int, c_vp],
     "tns_engine_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int,
                           ctypes.c_int, c_vpp],
     "tns_engine_set_chunk": [c_vp, ctypes.c_int64],

@@ -14,8 +14,11 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tnsgpu.h"
@@ -468,6 +471,243 @@ int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const*
             }
         }
         bt.run(as_stream(stream));
+    })
+}
+
+// ------------------------------------------------------------- batches
+// The same ops for many tensors at once: planning spread over host threads,
+// execution as one launch per kernel kind for all plans together.
+namespace {
+void parallel_items(int n, int threads, const std::function<void(int)>& fn) {
+    const int T = std::max(1, std::min(threads, n));
+    std::vector<std::exception_ptr> errs(static_cast<size_t>(T));
+    auto work = [&](int k) {
+        try {
+            for (int i = k; i < n; i += T) fn(i);
+        } catch (...) {
+            errs[static_cast<size_t>(k)] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int k = 1; k < T; k++) ts.emplace_back(work, k);
+    work(0);
+    for (auto& t : ts) t.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+void check_same_type(int n, tns_plan* const* plans) {
+    for (int i = 1; i < n; i++)
+        if (plans[i]->type != plans[0]->type) throw Error("a plan batch must hold one op type");
+}
+}  // namespace
+
+int tns_plan_contract_many(int n, const tns_shape* const* a, const tns_shape* const* b, int npairs,
+                           const int32_t* pairs, int threads, tns_plan** out) {
+    TNS_TRY({
+        std::vector<std::pair<int32_t, int32_t>> pr;
+        for (int i = 0; i < npairs; i++) pr.push_back({pairs[2 * i], pairs[2 * i + 1]});
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            parallel_items(n, threads, [&](int i) {
+                auto p = std::make_unique<tns_plan>();
+                p->type = tns_plan::CONTRACT;
+                plan_contract(a[i]->s, b[i]->s, pr, p->c);
+                out[i] = p.release();
+            });
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
+int tns_plan_qr_many(int n, const tns_shape* const* t, char direction, int nfree, const int32_t* free_names,
+                     int32_t cq, int32_t cr, int threads, tns_plan** out) {
+    TNS_TRY({
+        const auto fr = ids(nfree, free_names);
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            parallel_items(n, threads, [&](int i) {
+                auto p = std::make_unique<tns_plan>();
+                p->type = tns_plan::QR;
+                plan_qr(t[i]->s, direction, fr, cq, cr, p->q);
+                out[i] = p.release();
+            });
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
+int tns_plan_svd_many(int n, const tns_shape* const* t, int nfree, const int32_t* free_u, int32_t cu, int32_t cv,
+                      int32_t su, int32_t sv, int threads, tns_plan** out) {
+    TNS_TRY({
+        const auto fr = ids(nfree, free_u);
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            parallel_items(n, threads, [&](int i) {
+                auto p = std::make_unique<tns_plan>();
+                p->type = tns_plan::SVD;
+                plan_svd_factor(t[i]->s, fr, cu, cv, su, sv, p->s);
+                out[i] = p.release();
+            });
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
+int tns_plans_run(int n, tns_plan* const* plans, int dtype, const void* const* inputs, void* const* outputs,
+                  void* const* workspaces, void* stream) {
+    TNS_TRY({
+        if (n <= 0) return 0;
+        if (dtype != 0 && dtype != 1) throw Error("unsupported dtype");
+        check_same_type(n, plans);
+        const int64_t es = dtype == 0 ? 8 : 4;
+        const auto type = plans[0]->type;
+        std::vector<Layout> L(static_cast<size_t>(n));
+        for (int i = 0; i < n; i++) {
+            L[size_t(i)] = layout_of(plans[i], dtype);
+            if (L[size_t(i)].total > 0 && !workspaces[i]) throw Error("workspace required");
+        }
+        auto ws = [&](int i) { return static_cast<char*>(workspaces[i]); };
+        Batch bt(dtype);
+        if (type == tns_plan::CONTRACT) {
+            auto bufs = [&](int i, void** b) {
+                b[0] = const_cast<void*>(inputs[2 * i]);
+                b[1] = const_cast<void*>(inputs[2 * i + 1]);
+                b[2] = ws(i) + L[size_t(i)].off[2];
+                b[3] = ws(i) + L[size_t(i)].off[3];
+                b[4] = ws(i) + L[size_t(i)].off[4];
+                b[5] = outputs[i];
+            };
+            void* b[6];
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.perm(plans[i]->c.perm_a, b, es);
+                bt.perm(plans[i]->c.perm_b, b, es);
+            }
+            bt.flush_perm();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.fill(plans[i]->c.fills, b, es);
+            }
+            bt.flush_fill();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.gemm(plans[i]->c.gemms, b, es);
+            }
+            bt.flush_gemm();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.perm(plans[i]->c.perm_out, b, es);
+            }
+            bt.flush_perm();
+        } else if (type == tns_plan::QR) {
+            auto bufs = [&](int i, void** b) {
+                b[0] = const_cast<void*>(inputs[i]);
+                b[1] = ws(i) + L[size_t(i)].off[1];
+                b[2] = ws(i) + L[size_t(i)].off[2];
+                b[3] = ws(i) + L[size_t(i)].off[3];
+                b[4] = outputs[2 * i];
+                b[5] = outputs[2 * i + 1];
+            };
+            void* b[6];
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.perm(plans[i]->q.perm_in, b, es);
+            }
+            bt.flush_perm();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.qr(plans[i]->q.qrs, b, es, reinterpret_cast<double*>(ws(i) + L[size_t(i)].scratch));
+            }
+            bt.flush_qr();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.perm(plans[i]->q.perm_1, b, es);
+                bt.perm(plans[i]->q.perm_2, b, es);
+            }
+            bt.flush_perm();
+        } else if (type == tns_plan::SVD) {
+            auto bufs = [&](int i, void** b) {
+                b[0] = nullptr;
+                b[1] = ws(i) + L[size_t(i)].off[1];
+                b[2] = ws(i) + L[size_t(i)].off[2];
+                b[3] = ws(i) + L[size_t(i)].off[3];
+                b[4] = ws(i) + L[size_t(i)].off[4];
+                b[5] = outputs[3 * i];
+                b[6] = outputs[3 * i + 1];
+                b[7] = outputs[3 * i + 2];
+            };
+            void* b[8];
+            for (int i = 0; i < n; i++)
+                if (!plans[i]->factored) throw Error("call tns_svd_factor_many before tns_plans_run for svd");
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.fill(plans[i]->s.fills_s, b, es);
+            }
+            bt.flush_fill();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.perm(plans[i]->s.perm_u, b, es);
+                bt.perm(plans[i]->s.perm_v, b, es);
+            }
+            bt.flush_perm();
+            for (int i = 0; i < n; i++) {
+                bufs(i, b);
+                bt.diag(plans[i]->s.diag_s, b, es);
+            }
+            bt.flush_diag();
+        } else {
+            throw Error("tns_plans_run handles contract, qr and svd plans");
+        }
+        bt.run(as_stream(stream));
+    })
+}
+
+int tns_svd_factor_many(int n, tns_plan* const* plans, int dtype, const void* const* inputs,
+                        void* const* workspaces, int64_t remain_cut, double relative_cut, int threads, void* stream) {
+    TNS_TRY({
+        if (n <= 0) return 0;
+        for (int i = 0; i < n; i++)
+            if (plans[i]->type != tns_plan::SVD) throw Error("not an svd plan");
+        const int64_t es = dtype == 0 ? 8 : 4;
+        std::vector<Layout> L(static_cast<size_t>(n));
+        Batch bt(dtype);
+        for (int i = 0; i < n; i++) {
+            L[size_t(i)] = layout_of(plans[i], dtype);
+            char* ws = static_cast<char*>(workspaces[i]);
+            void* b[5] = {const_cast<void*>(inputs[i]), ws + L[size_t(i)].off[1], ws + L[size_t(i)].off[2],
+                          ws + L[size_t(i)].off[3], ws + L[size_t(i)].off[4]};
+            bt.perm(plans[i]->s.perm_in, b, es);
+        }
+        bt.flush_perm();
+        for (int i = 0; i < n; i++) {
+            char* ws = static_cast<char*>(workspaces[i]);
+            void* b[5] = {const_cast<void*>(inputs[i]), ws + L[size_t(i)].off[1], ws + L[size_t(i)].off[2],
+                          ws + L[size_t(i)].off[3], ws + L[size_t(i)].off[4]};
+            bt.svd(plans[i]->s.svds, b, es, reinterpret_cast<double*>(ws + L[size_t(i)].scratch));
+        }
+        bt.flush_svd();
+        hipStream_t st = as_stream(stream);
+        bt.run(st);
+        for (int i = 0; i < n; i++) {
+            auto* p = plans[i];
+            p->svals_host.assign(size_t(p->s.s_total), 0.0);
+            if (p->s.s_total)
+                check_hip(hipMemcpyAsync(p->svals_host.data(), static_cast<char*>(workspaces[i]) + L[size_t(i)].off[4],
+                                         size_t(p->s.s_total) * 8, hipMemcpyDeviceToHost, st),
+                          "singular values copy");
+        }
+        check_hip(hipStreamSynchronize(st), "svd synchronize");
+        parallel_items(n, threads, [&](int i) {
+            plan_svd_truncate(plans[i]->s, plans[i]->svals_host.data(), remain_cut, relative_cut);
+            plans[i]->factored = true;
+        });
     })
 }
 

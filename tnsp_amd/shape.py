@@ -199,9 +199,9 @@ NAMES = NameTable()
 class Shape:
     """Owning wrapper of a ``tns_shape`` handle plus its Python view."""
 
-    __slots__ = ("handle", "kind", "names", "edges", "size", "_blocks")
+    __slots__ = ("handle", "kind", "_names", "_edges", "size", "_blocks")
 
-    def __init__(self, handle, kind=None, names=None, edges=None):
+    def __init__(self, handle, kind=None, names=None, edges=None, lazy=False):
         self.handle = handle
         L = _lib.lib()
         k, rank, nseg_total, nblocks, size = (ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(),
@@ -213,25 +213,49 @@ class Shape:
         if kind is None:
             kind = next(kk for kk in KINDS.values() if kk.id == k.value)
         self.kind = kind
-        if names is None or edges is None:
-            r, ts = rank.value, nseg_total.value
-            nm = (ctypes.c_int32 * max(1, r))()
-            ns = (ctypes.c_int32 * max(1, r))()
-            sy = (ctypes.c_int32 * max(1, 2 * ts))()
-            dm = (ctypes.c_int64 * max(1, ts))()
-            ar = (ctypes.c_uint8 * max(1, r))()
-            _lib.check(L.tns_shape_export(handle, nm, ns, sy, dm, ar))
-            names, edges, s = [], [], 0
-            for i in range(r):
-                names.append(NAMES.name(nm[i]))
-                segs = []
-                for _ in range(ns[i]):
-                    q = tuple(sy[2 * s + c] for c in range(kind.length))
-                    segs.append((q, dm[s]))
-                    s += 1
-                edges.append(Edge(kind, segs, bool(ar[i])))
-        self.names = list(names)
-        self.edges = list(edges)
+        self._names = list(names) if names is not None else None
+        self._edges = list(edges) if edges is not None else None
+        if not lazy and (names is None or edges is None):
+            self._export()
+
+    @property
+    def names(self):
+        if self._names is None:
+            self._export()
+        return self._names
+
+    @property
+    def edges(self):
+        if self._edges is None:
+            self._export()
+        return self._edges
+
+    def _export(self):
+        """Names and edges read back from the handle."""
+        L = _lib.lib()
+        kind = self.kind
+        k, rank, nseg_total, nblocks, size = (ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64(),
+                                              ctypes.c_int64())
+        _lib.check(L.tns_shape_counts(self.handle, ctypes.byref(k), ctypes.byref(rank), ctypes.byref(nseg_total),
+                                      ctypes.byref(nblocks), ctypes.byref(size)))
+        r, ts = rank.value, nseg_total.value
+        nm = (ctypes.c_int32 * max(1, r))()
+        ns = (ctypes.c_int32 * max(1, r))()
+        sy = (ctypes.c_int32 * max(1, 2 * ts))()
+        dm = (ctypes.c_int64 * max(1, ts))()
+        ar = (ctypes.c_uint8 * max(1, r))()
+        _lib.check(L.tns_shape_export(self.handle, nm, ns, sy, dm, ar))
+        names, edges, s = [], [], 0
+        for i in range(r):
+            names.append(NAMES.name(nm[i]))
+            segs = []
+            for _ in range(ns[i]):
+                q = tuple(sy[2 * s + c] for c in range(kind.length))
+                segs.append((q, dm[s]))
+                s += 1
+            edges.append(Edge(kind, segs, bool(ar[i])))
+        self._names = list(names)
+        self._edges = list(edges)
 
     @classmethod
     def create(cls, kind, names, edges):
