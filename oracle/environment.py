@@ -1,0 +1,227 @@
+"""CPU oracle for tetragono's cached-environment ``replace`` and the sweep
+sampling that drives it.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Restates, on top of ``oracle.tat`` and ``oracle.peps`` (paths relative to the
+TNSP repository root, file ``tetragono/tetragono/auxiliaries/
+single_layer_auxiliaries.py`` unless named):
+
+* the four boundary MPS ``_up_to_down`` ("UDLR"), ``_down_to_up`` ("DULR"),
+  ``_left_to_right`` ("LRUD"), ``_right_to_left`` ("RLUD") (:402-432), each a
+  chain of ``_two_line_to_one_line`` (:463-498) from the ones row / column;
+* the inline environments within a row, ``_inline_left_to_right[_tailed]``
+  and ``_inline_right_to_left[_tailed]`` (:254-312), and within a column,
+  ``_inline_up_to_down[_tailed]`` and ``_inline_down_to_up[_tailed]``
+  (:314-372);
+* ``replace`` for two neighbouring sites (:593-708), horizontal pair
+  (:593-603, 641-651) and vertical pair (:652-687), the branches a sweep
+  reaches: the four-row variants read environments that nothing in a sweep
+  ever evaluates;
+* ``SweepSampling._single_term`` / ``__call__`` (sampling_lattice/
+  sampling.py:117-154) for the tV bond Hamiltonian, including its diagonal
+  target on a doubly occupied bond: ``Configuration.replace`` with no site
+  changed (sampling_lattice/lattice.py:231-267) returns ``hole(())``, the
+  amplitude of the current configuration (:500-510, 710-720).
+
+Environments are recomputed from the current lattice at each use: the lazy
+graph of the reference caches exactly these values and invalidates them when
+a site they depend on changes, so a from-scratch value is what every cache
+hit returns.
+"""
+import numpy as np
+
+from .peps import one, project, safe_contract, safe_rename, two_line_to_one_line
+
+
+class Environments:
+    """Environments of one projected lattice (``lat[l1][l2]``: shrunk site
+    tensors) at boundary-MPS bond dimension ``cut``."""
+
+    def __init__(self, lat, cut):
+        self.lat = [list(row) for row in lat]
+        self.L1, self.L2 = len(lat), len(lat[0])
+        self.cut = cut
+        self.one = one(lat[0][0].kind)
+
+    # -- boundary MPS (:402-432) ------------------------------------------------
+    def up_to_down(self, l1):
+        """_up_to_down[l1]: rows 0..l1 absorbed from the top; -1 is ones."""
+        mps = [self.one] * self.L2
+        for r in range(l1 + 1):
+            mps = two_line_to_one_line(mps, self.lat[r], self.cut, "UDLR")
+        return mps
+
+    def down_to_up(self, l1):
+        mps = [self.one] * self.L2
+        for r in reversed(range(l1, self.L1)):
+            mps = two_line_to_one_line(mps, self.lat[r], self.cut, "DULR")
+        return mps
+
+    def column(self, l2):
+        return [self.lat[l1][l2] for l1 in range(self.L1)]
+
+    def left_to_right(self, l2):
+        mps = [self.one] * self.L1
+        for c in range(l2 + 1):
+            mps = two_line_to_one_line(mps, self.column(c), self.cut, "LRUD")
+        return mps
+
+    def right_to_left(self, l2):
+        mps = [self.one] * self.L1
+        for c in reversed(range(l2, self.L2)):
+            mps = two_line_to_one_line(mps, self.column(c), self.cut, "RLUD")
+        return mps
+
+    # -- row inline environments (:254-312) --------------------------------------
+    def inline_left_tailed(self, l1, l2, up, down):
+        """_inline_left_to_right_tailed[l1, l2]; ``up`` = _up_to_down[l1-1],
+        ``down`` = _down_to_up[l1+1]."""
+        tailed = None
+        for c in range(-1, l2 + 1):
+            if c == -1:
+                left = self.one
+            else:
+                left = safe_contract(tailed, safe_rename(self.lat[l1][c], {"R": "R2"}), {("R2", "L"), ("D", "U")})
+                left = safe_contract(left, safe_rename(down[c], {"R": "R3"}), {("R3", "L"), ("D", "U")})
+            if c == self.L2 - 1:
+                tailed = left
+            else:
+                tailed = safe_rename(safe_contract(left, up[c + 1], {("R1", "L")}), {"R": "R1"})
+        return tailed
+
+    def inline_right(self, l1, l2, up, down, tailed_result=True):
+        """_inline_right_to_left_tailed[l1, l2] (or the untailed
+        _inline_right_to_left[l1, l2] with ``tailed_result=False``)."""
+        tailed = None
+        for c in reversed(range(l2, self.L2 + 1)):
+            if c == self.L2:
+                right = self.one
+            else:
+                right = safe_contract(tailed, safe_rename(self.lat[l1][c], {"L": "L2"}), {("L2", "R"), ("U", "D")})
+                right = safe_contract(right, safe_rename(up[c], {"L": "L1"}), {("L1", "R"), ("U", "D")})
+            if c == l2 and not tailed_result:
+                return right
+            if c == 0:
+                tailed = right
+            else:
+                tailed = safe_rename(safe_contract(right, down[c - 1], {("L3", "R")}), {"L": "L3"})
+        return tailed
+
+    # -- column inline environments (:314-372) -----------------------------------
+    def inline_up_tailed(self, l1, l2, left, right):
+        """_inline_up_to_down_tailed[l1, l2]; ``left`` = _left_to_right[l2-1],
+        ``right`` = _right_to_left[l2+1]."""
+        tailed = None
+        for r in range(-1, l1 + 1):
+            if r == -1:
+                inl = self.one
+            else:
+                inl = safe_contract(tailed, safe_rename(self.lat[r][l2], {"D": "D2"}), {("D2", "U"), ("R", "L")})
+                inl = safe_contract(inl, safe_rename(right[r], {"D": "D3"}), {("D3", "U"), ("R", "L")})
+            # r == L1 - 1 is never the argument of a tailed read in a sweep
+            tailed = safe_rename(safe_contract(inl, left[r + 1], {("D1", "U")}), {"D": "D1"})
+        return tailed
+
+    def inline_down_tailed(self, l1, l2, left, right):
+        """_inline_down_to_up_tailed[l1, l2]."""
+        tailed = None
+        for r in reversed(range(l1, self.L1 + 1)):
+            if r == self.L1:
+                inl = self.one
+            else:
+                inl = safe_contract(tailed, safe_rename(self.lat[r][l2], {"U": "U2"}), {("U2", "D"), ("L", "R")})
+                inl = safe_contract(inl, safe_rename(left[r], {"U": "U1"}), {("U1", "D"), ("L", "R")})
+            if r == 0:
+                tailed = inl
+            else:
+                tailed = safe_rename(safe_contract(inl, right[r - 1], {("U3", "D")}), {"U": "U3"})
+        return tailed
+
+    # -- replace (:593-708) -------------------------------------------------------
+    def replace_horizontal(self, l1, l2, t0, t1):
+        """<s'|psi> with sites (l1, l2), (l1, l2 + 1) replaced by t0, t1."""
+        up = self.up_to_down(l1 - 1)
+        down = self.down_to_up(l1 + 1)
+        left_part = self.inline_left_tailed(l1, l2 - 1, up, down)
+        left_dot = down[l2]
+        right_part = self.inline_right(l1, l2 + 2, up, down)
+        right_dot = up[l2 + 1]
+        res = safe_contract(left_part, safe_rename(t0, {"R": "R2"}), {("D", "U"), ("R2", "L")})
+        res = safe_contract(res, safe_rename(left_dot, {"R": "R3"}), {("D", "U"), ("R3", "L")})
+        res = safe_contract(res, safe_rename(right_dot, {"R": "R1"}), {("R1", "L")})
+        res = safe_contract(res, safe_rename(t1, {"R": "R2"}), {("D", "U"), ("R2", "L")})
+        return safe_contract(res, right_part, {("R1", "L1"), ("R2", "L2"), ("D", "U"), ("R3", "L3")})
+
+    def replace_vertical(self, l1, l2, t0, t1):
+        """<s'|psi> with sites (l1, l2), (l1 + 1, l2) replaced by t0, t1."""
+        left = self.left_to_right(l2 - 1)
+        right = self.right_to_left(l2 + 1)
+        up_part = self.inline_up_tailed(l1 - 1, l2, left, right)
+        up_dot = right[l1]
+        down_part = self.inline_down_tailed(l1 + 2, l2, left, right)
+        down_dot = left[l1 + 1]
+        res = safe_contract(up_part, safe_rename(t0, {"D": "D2"}), {("R", "L"), ("D2", "U")})
+        res = safe_contract(res, safe_rename(up_dot, {"D": "D3"}), {("R", "L"), ("D3", "U")})
+        res = safe_contract(res, safe_rename(down_dot, {"D": "D1"}), {("D1", "U")})
+        res = safe_contract(res, safe_rename(t1, {"D": "D2"}), {("R", "L"), ("D2", "U")})
+        return safe_contract(res, down_part, {("D1", "U1"), ("D2", "U2"), ("R", "L"), ("D3", "U3")})
+
+    def hole(self):
+        """hole(()) with hint ("H", 0): _inline_right_to_left[0, 0]."""
+        return self.inline_right(0, 0, [self.one] * self.L2, self.down_to_up(1), tailed_result=False)
+
+
+def value(t):
+    return float(t.data[0]) if t.size else 0.0
+
+
+def replace(sites, config, cut, positions, occupations):
+    """``Configuration.replace`` for a neighbouring pair ``positions`` =
+    ((l1, l2), (l1', l2')) set to ``occupations``: the result tensor."""
+    from .peps import shrink
+
+    (a1, a2), (b1, b2) = positions
+    env = Environments(project(sites, config), cut)
+    t0 = shrink(sites[a1][a2], a1, a2, occupations[0])
+    t1 = shrink(sites[b1][b2], b1, b2, occupations[1])
+    if a1 == b1 and b2 == a2 + 1:
+        return env.replace_horizontal(a1, a2, t0, t1)
+    if a2 == b2 and b1 == a1 + 1:
+        return env.replace_vertical(a1, a2, t0, t1)
+    raise ValueError("replace of this site pair is outside the sweep's bonds")
+
+
+def sweep(sites, config, cut, uniforms, alpha=1.0, t=1.0, V=1.0):
+    """One ``SweepSampling.__call__`` of the tV model (sampling.py:145-154):
+    bonds in ``bond_order``; on a bond with one particle the hop target, on a
+    doubly occupied bond (only the diagonal element when V != 0) the
+    configuration itself, whose ``replace`` is ``hole(())``; an empty bond has
+    no element and draws nothing.  ``uniforms[k]`` is the acceptance draw of
+    bond k.  Returns the final configuration, the per-bond trace of
+    (|ws| before, |wss| or None, accepted) and the final |ws|."""
+    from .sampling import bond_order
+
+    cfg = np.array(config, dtype=np.int64).copy()
+    env = Environments(project(sites, cfg), cut)
+    ws = value(env.hole())
+    trace = []
+    for k, (a, b) in enumerate(bond_order(len(cfg), len(cfg[0]))):
+        na, nb = int(cfg[a]), int(cfg[b])
+        if na == 0 and nb == 0 or (na == 1 and nb == 1 and V == 0):
+            trace.append((abs(ws), None, False))
+            continue
+        if na == nb:
+            wss = value(Environments(project(sites, cfg), cut).hole())
+            new = None
+        else:
+            new = (nb, na)
+            wss = value(replace(sites, cfg, cut, (a, b), new))
+        p = abs(wss / ws) ** (2 * alpha)  # one target before and after: n / n' = 1
+        acc = bool(uniforms[k] < p)
+        trace.append((abs(ws), abs(wss), acc))
+        if acc:
+            ws = wss
+            if new is not None:
+                cfg[a], cfg[b] = new
+    return cfg, trace, abs(ws)

@@ -1,0 +1,123 @@
+"""Oracle of the cached-environment ``replace`` (oracle/environment.py).
+
+Pins, on small FermiU1 lattices:
+* ``hole()`` through the row environments is the pinned amplitude
+  (oracle.peps.amplitude, itself checked against TAT's golden vectors), at a
+  truncating cut;
+* without truncation, ``replace`` of every horizontal and vertical bond
+  equals the amplitude of the changed configuration, sign included once both
+  one-element tensors are transposed to one edge order;
+* at a truncating cut, ``replace`` with the current sites is the same number
+  for every bond of a row (a horizontal strip contracted exactly between two
+  boundary MPS) and of a column; for row 0 it is ``hole(())``;
+* the sweep with an uncut boundary MPS makes the same decisions as the
+  Metropolis rule on exact amplitudes.
+"""
+import numpy as np
+import pytest
+
+from test_sampler import small_lattice
+
+
+def configs(L1, L2, N, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        flat = np.zeros(L1 * L2, dtype=np.int64)
+        flat[rng.choice(L1 * L2, N, replace=False)] = 1
+        out.append(flat.reshape(L1, L2))
+    return out
+
+
+def signed(t, names):
+    return float(t.transpose(names).data[0])
+
+
+@pytest.mark.parametrize("L1,L2,N,cut", [(3, 3, 4, 3), (4, 4, 8, 4)])
+def test_hole_is_amplitude(L1, L2, N, cut):
+    from oracle import environment as E, peps
+
+    _, _, sites = small_lattice(L1, L2, N, seed=1)
+    for cfg in configs(L1, L2, N, 2, seed=2):
+        lat = peps.project(sites, cfg)
+        a = peps.amplitude(lat, cut)
+        h = E.Environments(lat, cut).hole()
+        assert h.names == a.names
+        assert np.allclose(h.data, a.data, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("L1,L2,N", [(3, 3, 4), (3, 4, 5)])
+def test_replace_uncut_equals_amplitude(L1, L2, N):
+    from oracle import environment as E, peps, sampling
+
+    _, _, sites = small_lattice(L1, L2, N, seed=4)
+    cut = 10_000
+    (cfg,) = configs(L1, L2, N, 1, seed=5)
+    checked = 0
+    for a, b in sampling.bond_order(L1, L2):
+        if cfg[a] == cfg[b]:
+            continue
+        new = cfg.copy()
+        new[a], new[b] = cfg[b], cfg[a]
+        want = peps.amplitude(peps.project(sites, new), cut)
+        got = E.replace(sites, cfg, cut, (a, b), (int(new[a]), int(new[b])))
+        assert sorted(got.names) == sorted(want.names)
+        w = float(want.data[0])
+        assert abs(signed(got, want.names) - w) <= 1e-10 * abs(w), (a, b)
+        checked += 1
+    assert checked >= 4
+
+
+def test_replace_unchanged_is_row_and_column_invariant():
+    from oracle import environment as E, peps
+
+    L1, L2, N, cut = 4, 4, 8, 3
+    _, _, sites = small_lattice(L1, L2, N, seed=6)
+    (cfg,) = configs(L1, L2, N, 1, seed=7)
+    lat = peps.project(sites, cfg)
+    env = E.Environments(lat, cut)
+    hole = E.value(env.hole())
+    for l1 in range(L1):
+        row = [abs(E.value(env.replace_horizontal(l1, l2, lat[l1][l2], lat[l1][l2 + 1]))) for l2 in range(L2 - 1)]
+        assert np.allclose(row, row[0], rtol=1e-11, atol=0), l1
+        if l1 == 0:
+            assert abs(row[0] - abs(hole)) <= 1e-12 * abs(hole)
+    cols = []
+    for l2 in range(L2):
+        col = [abs(E.value(env.replace_vertical(l1, l2, lat[l1][l2], lat[l1 + 1][l2]))) for l1 in range(L1 - 1)]
+        assert np.allclose(col, col[0], rtol=1e-11, atol=0), l2
+        cols.append(col[0])
+    # at this cut the column contractions approximate the amplitude differently
+    # from the row ones: a real truncation, so the invariances above are not
+    # the uncut identity in disguise
+    assert not np.allclose(cols, abs(hole), rtol=1e-3)
+
+
+def test_sweep_uncut_follows_exact_metropolis():
+    from oracle import environment as E, peps, sampling
+
+    L1, L2, N = 3, 3, 4
+    _, _, sites = small_lattice(L1, L2, N, seed=8)
+    cut = 10_000
+    (cfg,) = configs(L1, L2, N, 1, seed=9)
+    bonds = sampling.bond_order(L1, L2)
+    u = np.random.default_rng(10).random(len(bonds))
+    final, trace, last = E.sweep(sites, cfg, cut, u)
+    cur = cfg.copy()
+    psi = abs(peps.psi(sites, cur, cut)[0])
+    for k, (a, b) in enumerate(bonds):
+        before, wss, acc = trace[k]
+        assert abs(before - psi) <= 1e-10 * psi
+        if cur[a] == cur[b]:
+            if cur[a] == 1:
+                assert abs(wss - psi) <= 1e-10 * psi
+            continue
+        new = cur.copy()
+        new[a], new[b] = cur[b], cur[a]
+        p_new = abs(peps.psi(sites, new, cut)[0])
+        assert abs(wss - p_new) <= 1e-10 * p_new
+        assert acc == (u[k] < (p_new / psi) ** 2)
+        if acc:
+            cur, psi = new, p_new
+    assert np.array_equal(final, cur)
+    assert abs(last - psi) <= 1e-10 * psi

@@ -97,6 +97,11 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libtnsgpu.so not built ({LIB_PATH}); run tnsp_amd.build.build()")
+    # torch ships its own HIP runtime under the same soname as the system
+    # one; loading torch first makes the library bind to that copy, so the
+    # process holds one runtime whichever module the caller imported first
+    import torch  # noqa: F401
+
     L = ctypes.CDLL(LIB_PATH)
     for name, args in _SIGNATURES.items():
         f = getattr(L, name)

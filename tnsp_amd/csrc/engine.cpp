@@ -546,8 +546,13 @@ void Engine::project_sites(hipStream_t st) {
         a[i].arena = A_PERSIST;
         a[i].off = roff[i / nphys_];
     }
+    // the projection selects elements: it runs in the storage type whatever
+    // the GEMM precision, so projected sites equal the raw site values
     const std::vector<std::pair<int32_t, int32_t>> pr{{names_[N_P0], names_[N_Q]}};
+    const int prec = gemm_prec_;
+    gemm_prec_ = 0;
     contract(pa, pb, pr, po, A_PERSIST, st);
+    gemm_prec_ = prec;
     keep_[A_PERSIST] = arenas_[A_PERSIST].used;
     proj_.assign(o.begin(), o.end());
 }

@@ -1,0 +1,232 @@
+"""Sweep sampling with cached environments, many chains in lockstep.
+
+tetragono evaluates a proposal psi(s') with ``Configuration.replace``
+(tetragono/tetragono/auxiliaries/single_layer_auxiliaries.py:500-708, paths
+relative to the TNSP repository root): only the two changed sites are
+contracted, between environments cached from earlier proposals.  Along a
+sweep (sampling_lattice/sampling.py:145-190: horizontal bonds row by row,
+then vertical bonds column by column) the caches that ``replace`` reads are:
+
+* horizontal pass, row l1: the boundary MPS of the rows above
+  (``_up_to_down[l1-1]``, rebuilt after each row) and below
+  (``_down_to_up[l1+1]``, built once at the start of the sweep), the inline
+  environment right of the bond (``_inline_right_to_left_tailed``, built
+  once per row from the row as it was when the row started: the columns
+  right of the bond are unchanged until the sweep reaches them) and the one
+  left of it (``_inline_left_to_right_tailed``, extended by one column after
+  each bond with the column as decided);
+* vertical pass, column l2: the same with the column MPS ``_left_to_right``
+  / ``_right_to_left`` and the column inline environments.
+
+``EnvironmentSampler`` keeps exactly those tensors for every chain, as
+batches (``tnsp_amd.batch``): each contraction, QR and SVD of a step runs
+for all chains in one launch per kernel kind.  The results equal the lazy
+graph's, which recomputes an invalidated environment from the same inputs.
+
+A doubly occupied bond of the tV model has only its diagonal element, so
+``_single_term`` proposes the configuration itself; ``replace`` with no site
+changed is ``hole(())`` (sampling_lattice/lattice.py:231-267), the
+amplitude computed afresh, and the draw compares it with the carried psi.
+``diagonal="hole"`` does that through the amplitude engine;
+``diagonal="keep"`` skips it (the two amplitudes differ only by the
+boundary-MPS truncation, so the step accepts with probability one up to
+that difference).
+"""
+import numpy as np
+
+from . import batch as Bt
+from .peps import _one
+from .sampler import bond_order
+
+
+class EnvironmentSampler:
+    """Lockstep sweep sampling of B chains of a FermiU1 or FermiZ2 PEPS with
+    the tV Hamiltonian (hops t, density interaction V)."""
+
+    def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole"):
+        if diagonal not in ("hole", "keep"):
+            raise ValueError('diagonal must be "hole" or "keep"')
+        self.engine = engine
+        self.lattice = engine.lattice
+        spec = engine.spec
+        self.L1, self.L2 = spec.L1, spec.L2
+        self.cut = spec.cut if cut is None else cut
+        self.t, self.V, self.alpha = float(t), float(V), float(alpha)
+        self.diagonal = diagonal
+        self.configs = np.ascontiguousarray(np.asarray(configs), dtype=np.int8).copy()
+        if self.configs.ndim != 3 or self.configs.shape[1:] != (self.L1, self.L2):
+            raise ValueError("configs must have shape (chains, L1, L2)")
+        self.gen = np.random.Generator(np.random.PCG64(seed))
+        self.bonds = bond_order(self.L1, self.L2)
+        self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
+                       for l2 in range(self.L2)] for l1 in range(self.L1)]
+        self._one = _one(spec, self.lattice.dtype)
+        self.psi = np.abs(engine.evaluate(self.configs, cut=self.cut))
+        if np.any(self.psi == 0):
+            raise ValueError("initial configurations with psi = 0")
+        self.proposals = 0
+        self.accepted = 0
+        self.replaced = 0
+        self.refreshed = 0
+
+    # -- batches of site tensors --------------------------------------------------
+    def _site(self, l1, l2, occ):
+        """Projected site (l1, l2) for occupations ``occ`` (one per item)."""
+        pair = self._proj[l1][l2]
+        t0 = pair[0]
+        shapes = [pair[o]._shape for o in occ]
+        data = [pair[o]._data for o in occ]
+        return Bt.TensorBatch(t0.model, t0.dtype, t0.names, shapes, data)
+
+    def _row(self, l1):
+        return [self._site(l1, l2, self.configs[:, l1, l2]) for l2 in range(self.L2)]
+
+    def _column(self, l2):
+        return [self._site(l1, l2, self.configs[:, l1, l2]) for l1 in range(self.L1)]
+
+    def _ones(self, n):
+        return Bt.TensorBatch.repeat(self._one, n)
+
+    # -- boundary MPS (single_layer_auxiliaries.py:463-498) -------------------------
+    def _two_line(self, udlr, line_1, line_2):
+        up, down, left, right = udlr
+        l1n, l2n, r1n, r2n = left + "1", left + "2", right + "1", right + "2"
+        n = len(line_1)
+        dl = [Bt.contract(line_1[i].rename({left: l1n, right: r1n}), line_2[i].rename({left: l2n, right: r2n}),
+                          [(down, up)]) for i in range(n)]
+        for i in range(n - 1):
+            q, r = Bt.qr(dl[i], "r", [x for x in (r1n, r2n) if x in dl[i].names], right, left)
+            dl[i] = q
+            dl[i + 1] = Bt.contract(dl[i + 1], r, [(l1n, r1n), (l2n, r2n)])
+        for i in reversed(range(1, n)):
+            u, s, v = Bt.svd(dl[i], [left], right, left, left, right, self.cut)
+            dl[i] = v
+            dl[i - 1] = Bt.contract(Bt.contract(dl[i - 1], u, [(right, left)]), s, [(right, left)])
+        return dl
+
+    # -- one bond ---------------------------------------------------------------------
+    def _term(self, a, b, replace):
+        """``_single_term`` on bond (a, b) for every chain: one uniform per
+        chain; chains with one particle on the bond propose the hop and read
+        psi(s') from ``replace(idx, t0, t1)``."""
+        B = len(self.configs)
+        na = self.configs[:, a[0], a[1]].copy()
+        nb = self.configs[:, b[0], b[1]].copy()
+        u = self.gen.random(B)
+        hop = np.nonzero(na != nb)[0]
+        dbl = np.nonzero((na == 1) & (nb == 1))[0] if self.V != 0 else np.zeros(0, dtype=np.int64)
+        if len(hop):
+            t0 = self._site(a[0], a[1], nb[hop])
+            t1 = self._site(b[0], b[1], na[hop])
+            wss = np.abs(replace(hop, t0, t1).values())
+            self.replaced += len(hop)
+            self.proposals += len(hop)
+            acc = u[hop] < (wss / self.psi[hop]) ** (2 * self.alpha)
+            take = hop[acc]
+            self.configs[take, a[0], a[1]], self.configs[take, b[0], b[1]] = nb[take], na[take]
+            self.psi[take] = wss[acc]
+            self.accepted += int(acc.sum())
+        if len(dbl) and self.diagonal == "hole":
+            wss = np.abs(self.engine.evaluate(self.configs[dbl], cut=self.cut))
+            self.refreshed += len(dbl)
+            acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
+            self.psi[dbl[acc]] = wss[acc]
+
+    # -- passes -----------------------------------------------------------------------
+    def _horizontal(self, log=None):
+        L1, L2 = self.L1, self.L2
+        B = len(self.configs)
+        one = self._ones(B)
+        down = {L1: [one] * L2}
+        for l1 in reversed(range(1, L1)):
+            down[l1] = self._two_line("DULR", down[l1 + 1], self._row(l1))
+        up = [one] * L2
+        for l1 in range(L1):
+            dn = down[l1 + 1]
+            row = self._row(l1)
+            # _inline_right_to_left_tailed[l1, c] for c = L2 .. 0 (:283-312)
+            rt, tailed = {}, None
+            for c in reversed(range(L2 + 1)):
+                if c == L2:
+                    right = one
+                else:
+                    right = Bt.contract(tailed, row[c].rename({"L": "L2"}), [("L2", "R"), ("U", "D")])
+                    right = Bt.contract(right, up[c].rename({"L": "L1"}), [("L1", "R"), ("U", "D")])
+                tailed = right if c == 0 else Bt.contract(right, dn[c - 1], [("L3", "R")]).rename({"L": "L3"})
+                rt[c] = tailed
+            if l1 == 0:
+                # hole(()) = _inline_right_to_left[0, 0]: psi at the start of the sweep
+                self.psi = np.abs(rt[0].values())
+            # _inline_left_to_right_tailed[l1, -1] (:254-282)
+            lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
+            for l2 in range(L2 - 1):
+                def replace(idx, t0, t1, lt=lt, l2=l2):
+                    # single_layer_auxiliaries.py:641-651
+                    res = Bt.contract(lt.take(idx), t0.rename({"R": "R2"}), [("D", "U"), ("R2", "L")])
+                    res = Bt.contract(res, dn[l2].take(idx).rename({"R": "R3"}), [("D", "U"), ("R3", "L")])
+                    res = Bt.contract(res, up[l2 + 1].take(idx).rename({"R": "R1"}), [("R1", "L")])
+                    res = Bt.contract(res, t1.rename({"R": "R2"}), [("D", "U"), ("R2", "L")])
+                    return Bt.contract(res, rt[l2 + 2].take(idx), [("R1", "L1"), ("R2", "L2"), ("D", "U"),
+                                                                    ("R3", "L3")])
+
+                self._term((l1, l2), (l1, l2 + 1), replace)
+                site = self._site(l1, l2, self.configs[:, l1, l2])
+                left = Bt.contract(lt, site.rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
+                left = Bt.contract(left, dn[l2].rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
+                lt = Bt.contract(left, up[l2 + 1], [("R1", "L")]).rename({"R": "R1"})
+            if l1 < L1 - 1:
+                up = self._two_line("UDLR", up, self._row(l1))
+            if log:
+                log(f"row {l1} done")
+
+    def _vertical(self, log=None):
+        L1, L2 = self.L1, self.L2
+        B = len(self.configs)
+        one = self._ones(B)
+        rcol = {L2: [one] * L1}
+        for l2 in reversed(range(1, L2)):
+            rcol[l2] = self._two_line("RLUD", rcol[l2 + 1], self._column(l2))
+        lcol = [one] * L1
+        for l2 in range(L2):
+            rc = rcol[l2 + 1]
+            col = self._column(l2)
+            # _inline_down_to_up_tailed[r, l2] for r = L1 .. 0 (:343-372)
+            dt, tailed = {}, None
+            for r in reversed(range(L1 + 1)):
+                if r == L1:
+                    inl = one
+                else:
+                    inl = Bt.contract(tailed, col[r].rename({"U": "U2"}), [("U2", "D"), ("L", "R")])
+                    inl = Bt.contract(inl, lcol[r].rename({"U": "U1"}), [("U1", "D"), ("L", "R")])
+                tailed = inl if r == 0 else Bt.contract(inl, rc[r - 1], [("U3", "D")]).rename({"U": "U3"})
+                dt[r] = tailed
+            # _inline_up_to_down_tailed[-1, l2] (:314-342)
+            ut = Bt.contract(one, lcol[0], [("D1", "U")]).rename({"D": "D1"})
+            for l1 in range(L1 - 1):
+                def replace(idx, t0, t1, ut=ut, l1=l1):
+                    # single_layer_auxiliaries.py:652-687
+                    res = Bt.contract(ut.take(idx), t0.rename({"D": "D2"}), [("R", "L"), ("D2", "U")])
+                    res = Bt.contract(res, rc[l1].take(idx).rename({"D": "D3"}), [("R", "L"), ("D3", "U")])
+                    res = Bt.contract(res, lcol[l1 + 1].take(idx).rename({"D": "D1"}), [("D1", "U")])
+                    res = Bt.contract(res, t1.rename({"D": "D2"}), [("R", "L"), ("D2", "U")])
+                    return Bt.contract(res, dt[l1 + 2].take(idx), [("D1", "U1"), ("D2", "U2"), ("R", "L"),
+                                                                    ("D3", "U3")])
+
+                self._term((l1, l2), (l1 + 1, l2), replace)
+                site = self._site(l1, l2, self.configs[:, l1, l2])
+                inl = Bt.contract(ut, site.rename({"D": "D2"}), [("D2", "U"), ("R", "L")])
+                inl = Bt.contract(inl, rc[l1].rename({"D": "D3"}), [("D3", "U"), ("R", "L")])
+                ut = Bt.contract(inl, lcol[l1 + 1], [("D1", "U")]).rename({"D": "D1"})
+            if l2 < L2 - 1:
+                lcol = self._two_line("LRUD", lcol, self._column(l2))
+            if log:
+                log(f"column {l2} done")
+
+    def sweep(self, log=None):
+        """One ``SweepSampling.__call__`` for every chain; returns the
+        configurations.  ``self.psi`` holds |psi| as carried by the sweep
+        (the last accepted ``replace`` value of each chain).  ``log``, if
+        given, is called with a progress line after every row and column."""
+        self._horizontal(log)
+        self._vertical(log)
+        return self.configs

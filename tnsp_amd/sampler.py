@@ -188,14 +188,18 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
     ``energy()`` (per-chain local energies) after each timed advance; returns
     ESS of the energy estimator, the sampling seconds (energy evaluation not
     included), the energy per site and tau."""
-    for _ in range(burn_in):
+    for i in range(burn_in):
         advance()
+        if log:
+            log(f"burn-in {i + 1}/{burn_in}")
     energies = []
     t_sample = 0.0
     for _ in range(samples):
         t0 = time.time()
         advance()
         t_sample += time.time() - t0
+        if log:
+            log(f"sample {len(energies) + 1}/{samples} drawn, {t_sample:.1f}s sampling")
         energies.append(energy())
         if log:
             log(f"{len(energies)}/{samples} samples, {t_sample:.1f}s sampling")
