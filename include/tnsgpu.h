@@ -105,6 +105,16 @@ int tns_plans_run(int n, tns_plan* const* plans, int dtype, const void* const* i
                   void* const* workspaces, void* stream);
 int tns_svd_factor_many(int n, tns_plan* const* plans, int dtype, const void* const* inputs,
                         void* const* workspaces, int64_t remain_cut, double relative_cut, int threads, void* stream);
+/* item-wise helpers of the batched ops (no reference counterpart: the
+ * reference handles one tensor per call): free n handles; edge_rename of n
+ * shapes with one mapping (tensor.hpp:760-784); output shape `which` of n
+ * plans with its element count; workspace bytes of n plans. */
+int tns_shapes_free(int n, tns_shape* const* shapes);
+int tns_plans_free(int n, tns_plan* const* plans);
+int tns_shape_rename_many(int n, const tns_shape* const* shapes, int nren, const int32_t* old_names,
+                          const int32_t* new_names, tns_shape** out);
+int tns_plans_output_shapes(int n, tns_plan* const* plans, int which, tns_shape** out, int64_t* sizes);
+int tns_plans_workspace(int n, tns_plan* const* plans, int dtype, int64_t* bytes);
 
 /* batched amplitude engine: psi(s) of many configurations of one PEPS in
  * lockstep (tetragono's single-layer boundary MPS,

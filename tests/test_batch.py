@@ -27,15 +27,34 @@ def shapes(kind, n, seed=0):
     return sa, sb
 
 
-def same_plan(p, q, nout):
+def same_plan(handle, q, nout):
+    """Plan ``handle`` (owned by a batch) against the single plan ``q``."""
+    import ctypes
+
+    from tnsp_amd import _lib
+    from tnsp_amd.plan import DTYPE_CODE
+    from tnsp_amd.shape import Shape
+
+    L = _lib.lib()
+    h = ctypes.c_void_p(int(handle))
     for w in range(nout):
-        x, y = p.output(w), q.output(w)
+        o = ctypes.c_void_p()
+        _lib.check(L.tns_plan_output_shape(h, w, ctypes.byref(o)))
+        x, y = Shape(o, q.kind), q.output(w)
         assert x.names == y.names
         assert x.edges == y.edges
         assert x.size == y.size
     for dt in ("float64", "float32"):
-        assert p.workspace(dt) == q.workspace(dt)
-    assert p.flops() == q.flops()
+        b = ctypes.c_int64()
+        _lib.check(L.tns_plan_workspace(h, DTYPE_CODE[dt], ctypes.byref(b)))
+        assert b.value == q.workspace(dt)
+    f = ctypes.c_int64()
+    _lib.check(L.tns_plan_flops(h, ctypes.byref(f)))
+    assert f.value == q.flops()
+
+
+def handles(shapes):
+    return [s.handle.value for s in shapes]
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -44,8 +63,8 @@ def test_contract_many_equals_single(kind):
 
     sa, sb = shapes(kind, 9)
     pairs = [("k", "k"), ("l", "l")]
-    many = batch.plan_contract(sa, sb, pairs)
-    for a, b, p in zip(sa, sb, many):
+    many = batch.plan_contract(handles(sa), handles(sb), pairs)
+    for a, b, p in zip(sa, sb, many.arr):
         same_plan(p, plan.contract(a, b, pairs), 1)
 
 
@@ -55,8 +74,8 @@ def test_qr_many_equals_single(kind, direction):
     from tnsp_amd import batch, plan
 
     sa, _ = shapes(kind, 7, seed=2)
-    many = batch.plan_qr(sa, direction, ["i", "k"], "cq", "cr")
-    for a, p in zip(sa, many):
+    many = batch.plan_qr(handles(sa), direction, ["i", "k"], "cq", "cr")
+    for a, p in zip(sa, many.arr):
         same_plan(p, plan.qr(a, direction, ["i", "k"], "cq", "cr"), 2)
 
 
@@ -65,8 +84,8 @@ def test_svd_many_equals_single(kind):
     from tnsp_amd import batch, plan
 
     sa, _ = shapes(kind, 7, seed=4)
-    many = batch.plan_svd(sa, ["j"], "cu", "cv", "su", "sv")
-    for a, p in zip(sa, many):
+    many = batch.plan_svd(handles(sa), ["j"], "cu", "cv", "su", "sv")
+    for a, p in zip(sa, many.arr):
         # output structures depend on the kept singular values: known after the
         # factor phase on the device
         same_plan(p, plan.svd(a, ["j"], "cu", "cv", "su", "sv"), 0)
@@ -79,4 +98,4 @@ def test_many_reports_item_error():
     # item 3 contracts two edges of equal arrow: not conjugate, rejected
     sb[3] = sa[3].rename({"i": "m", "j": "n"})
     with pytest.raises(RuntimeError):
-        batch.plan_contract(sa, sb, [("k", "k"), ("l", "l")])
+        batch.plan_contract(handles(sa), handles(sb), [("k", "k"), ("l", "l")])

@@ -92,7 +92,7 @@ def test_batch_svd(kind, cut):
         assert rel(storage(r1), r0.data) < 1e-10, i
 
 
-def test_batch_rename_take_put_values():
+def test_batch_rename_take_select_values():
     from tnsp_amd import batch
 
     pairs = items("FermiU1", 4, seed=11)
@@ -104,9 +104,11 @@ def test_batch_rename_take_put_values():
         assert np.array_equal(storage(rn.tensor(i)), a.data)
     sub = ga.take([3, 1])
     assert np.array_equal(storage(sub.tensor(0)), pairs[3][0].data)
-    merged = ga.put([0], sub.take([0]))
-    assert np.array_equal(storage(merged.tensor(0)), pairs[3][0].data)
-    assert np.array_equal(storage(merged.tensor(1)), pairs[1][0].data)
+    assert np.array_equal(storage(sub.tensor(1)), pairs[1][0].data)
+    # select: item i is choices[which[i]]
+    sel = batch.TensorBatch.select([ga.tensor(0), ga.tensor(2)], [1, 0, 1])
+    for i, w in enumerate([2, 0, 2]):
+        assert np.array_equal(storage(sel.tensor(i)), pairs[w][0].data)
     # full contraction of each A with its conjugate: one-element tensors
     ca = batch.TensorBatch.from_tensors([to_gpu(a.conjugate()) for a, _ in pairs])
     full = batch.contract(ga, ca, [(n, n) for n in "ijkl"])

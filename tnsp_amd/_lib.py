@@ -57,6 +57,11 @@ _SIGNATURES = {
     "tns_plans_run": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, c_vpp, c_vp],
     "tns_svd_factor_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int64, ctypes.c_double,
                             ctypes.c_int, c_vp],
+    "tns_shapes_free": [ctypes.c_int, c_vpp],
+    "tns_plans_free": [ctypes.c_int, c_vpp],
+    "tns_shape_rename_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_i32p, c_i32p, c_vpp],
+    "tns_plans_output_shapes": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_i64p],
+    "tns_plans_workspace": [ctypes.c_int, c_vpp, ctypes.c_int, c_i64p],
     "tns_engine_create": [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int,
                           ctypes.c_int, c_vpp],
     "tns_engine_set_chunk": [c_vp, ctypes.c_int64],

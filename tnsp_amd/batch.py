@@ -4,10 +4,15 @@ A :class:`TensorBatch` holds n tensors that share names, symmetry model and
 dtype but differ in block structure and data (the same tensor of n sampled
 configurations).  ``contract`` / ``qr`` / ``svd`` plan all n on host threads
 and run them as one launch per kernel kind (``tns_plan_*_many``,
-``tns_plans_run``), which is how the environment-based amplitude updates of
-``tnsp_amd.auxiliaries`` run for many chains at once.  Semantics per tensor
-are those of the drop-in ``Tensor`` methods (PyTAT ``contract`` / ``qr`` /
+``tns_plans_run``), which is how the environment-based sweep of
+``tnsp_amd.environment`` runs many chains at once.  Semantics per tensor are
+those of the drop-in ``Tensor`` methods (PyTAT ``contract`` / ``qr`` /
 ``svd``, PyTAT/PyTAT.hpp:988-1074).
+
+A batch is flat arrays, not per-item Python objects: native shape handles,
+device pointers and element counts, plus the torch buffers the pointers live
+in.  Outputs of one op share one buffer; ``tensor(i)`` views one item as a
+drop-in ``Tensor``.
 """
 import ctypes
 import os
@@ -15,60 +20,100 @@ import os
 import numpy as np
 
 from . import _lib
-from .plan import DTYPE_CODE, Plan, cut_params
+from .plan import DTYPE_CODE, cut_params
 from .shape import NAMES, Shape
 from .tensor import Tensor, _stream, _tdtype, _torch, device
+
+_ES = {"float64": 8, "float32": 4}
 
 
 def _threads():
     return int(os.environ.get("TNS_THREADS", min(16, os.cpu_count() or 1)))
 
 
-def _vp_array(values):
-    return (ctypes.c_void_p * max(1, len(values)))(*[int(v) for v in values])
+def _vpp(arr):
+    return arr.ctypes.data_as(_lib.c_vpp)
+
+
+def _i64p(arr):
+    return arr.ctypes.data_as(_lib.c_i64p)
+
+
+class _Owned:
+    """Native handles (shapes or plans) released together."""
+    __slots__ = ("arr", "free")
+
+    def __init__(self, arr, free):
+        self.arr = arr
+        self.free = free
+
+    def __del__(self):
+        if _lib._lib is not None and len(self.arr):
+            getattr(_lib._lib, self.free)(len(self.arr), _vpp(self.arr))
 
 
 class TensorBatch:
-    __slots__ = ("model", "dtype", "names", "shapes", "data")
+    """n tensors of one name list.  ``handles`` (uint64: native shape
+    handles), ``ptrs`` (uint64: device addresses), ``sizes`` (int64:
+    elements), ``bufs`` / ``buf_idx`` / ``offs``: the torch buffer and
+    element offset of each item; ``keep``: owners of the handles."""
+    __slots__ = ("model", "dtype", "names", "handles", "ptrs", "sizes", "bufs", "buf_idx", "offs", "keep")
 
-    def __init__(self, model, dtype, names, shapes, data):
+    def __init__(self, model, dtype, names, handles, sizes, bufs, buf_idx, offs, keep):
         self.model = model
         self.dtype = dtype
         self.names = list(names)
-        self.shapes = shapes
-        self.data = data
+        self.handles = np.ascontiguousarray(handles, dtype=np.uint64)
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        self.bufs = bufs
+        self.buf_idx = np.ascontiguousarray(buf_idx, dtype=np.int64)
+        self.offs = np.ascontiguousarray(offs, dtype=np.int64)
+        self.keep = keep
+        base = np.array([b.data_ptr() for b in bufs], dtype=np.uint64)
+        self.ptrs = base[self.buf_idx] + self.offs.astype(np.uint64) * np.uint64(_ES[dtype])
 
+    # -- construction ---------------------------------------------------------------
     @classmethod
     def from_tensors(cls, tensors):
         t0 = tensors[0]
         for t in tensors:
             if t.names != t0.names or t.model is not t0.model or t.dtype != t0.dtype:
                 raise ValueError("a tensor batch needs one name list, model and dtype")
-        return cls(t0.model, t0.dtype, t0.names, [t._shape for t in tensors], [t._data for t in tensors])
+        n = len(tensors)
+        return cls(t0.model, t0.dtype, t0.names, [t._shape.handle.value for t in tensors],
+                   [t._shape.size for t in tensors], [t._data for t in tensors], np.arange(n), np.zeros(n),
+                   tuple(tensors))
+
+    @classmethod
+    def select(cls, choices, which):
+        """Item i is ``choices[which[i]]`` (tensors of one name list): the
+        same site tensor at the occupation each configuration holds."""
+        t0 = choices[0]
+        which = np.asarray(which, dtype=np.int64)
+        h = np.array([t._shape.handle.value for t in choices], dtype=np.uint64)
+        sz = np.array([t._shape.size for t in choices], dtype=np.int64)
+        return cls(t0.model, t0.dtype, t0.names, h[which], sz[which], [t._data for t in choices], which,
+                   np.zeros(len(which)), tuple(choices))
 
     @classmethod
     def repeat(cls, tensor, n):
-        return cls(tensor.model, tensor.dtype, tensor.names, [tensor._shape] * n, [tensor._data] * n)
+        return cls.select([tensor], np.zeros(n, dtype=np.int64))
 
     def __len__(self):
-        return len(self.shapes)
+        return len(self.handles)
 
     def tensor(self, i):
-        return Tensor._wrap(self.model, self.dtype, self.shapes[i], self.data[i])
+        """Item i as a drop-in ``Tensor`` (shape copied, storage shared)."""
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().tns_shape_rename(ctypes.c_void_p(int(self.handles[i])), 0, _lib.arr32([]),
+                                               _lib.arr32([]), ctypes.byref(h)))
+        o, n = int(self.offs[i]), int(self.sizes[i])
+        return Tensor._wrap(self.model, self.dtype, Shape(h, self.model.kind), self.bufs[self.buf_idx[i]][o:o + n])
 
     def take(self, idx):
-        return TensorBatch(self.model, self.dtype, self.names, [self.shapes[i] for i in idx],
-                           [self.data[i] for i in idx])
-
-    def put(self, idx, other):
-        """This batch with items ``idx`` replaced by ``other``'s (same names)."""
-        if other.names != self.names:
-            raise ValueError("put needs the same names")
-        shapes, data = list(self.shapes), list(self.data)
-        for j, i in enumerate(idx):
-            shapes[i] = other.shapes[j]
-            data[i] = other.data[j]
-        return TensorBatch(self.model, self.dtype, self.names, shapes, data)
+        idx = np.asarray(idx, dtype=np.int64)
+        return TensorBatch(self.model, self.dtype, self.names, self.handles[idx], self.sizes[idx], self.bufs,
+                           self.buf_idx[idx], self.offs[idx], self.keep)
 
     def rename(self, mapping):
         """safe_rename (tetragono/tetragono/utility.py:353-358): names absent
@@ -77,139 +122,135 @@ class TensorBatch:
         if not m:
             return self
         old = list(m)
-        o_ids = _lib.arr32(NAMES.ids(old))
-        n_ids = _lib.arr32(NAMES.ids([m[k] for k in old]))
-        L = _lib.lib()
-        shapes = []
-        for s in self.shapes:
-            h = ctypes.c_void_p()
-            _lib.check(L.tns_shape_rename(s.handle, len(old), o_ids, n_ids, ctypes.byref(h)))
-            shapes.append(Shape(h, self.model.kind, lazy=True))
-        return TensorBatch(self.model, self.dtype, [m.get(n, n) for n in self.names], shapes, self.data)
+        out = np.zeros(len(self), dtype=np.uint64)
+        _lib.check(_lib.lib().tns_shape_rename_many(len(self), _vpp(self.handles), len(old),
+                                                    _lib.arr32(NAMES.ids(old)),
+                                                    _lib.arr32(NAMES.ids([m[k] for k in old])), _vpp(out)))
+        return TensorBatch(self.model, self.dtype, [m.get(n, n) for n in self.names], out, self.sizes, self.bufs,
+                           self.buf_idx, self.offs, (_Owned(out, "tns_shapes_free"), self.keep))
 
     def values(self):
         """First storage element of every item (0 for an empty tensor): the
         value of one-element tensors."""
-        torch = _torch()
         out = np.zeros(len(self))
-        nz = [i for i, d in enumerate(self.data) if d.numel() > 0]
-        if nz:
-            out[nz] = torch.stack([self.data[i][0] for i in nz]).double().cpu().numpy()
+        for b in np.unique(self.buf_idx):
+            sel = np.nonzero((self.buf_idx == b) & (self.sizes > 0))[0]
+            if len(sel):
+                idx = _torch().as_tensor(self.offs[sel], device=self.bufs[b].device)
+                out[sel] = self.bufs[b][idx].double().cpu().numpy()
         return out
 
 
+# -- plans ------------------------------------------------------------------------------
+def _plans(n, fn, *args):
+    out = np.zeros(n, dtype=np.uint64)
+    _lib.check(fn(n, *args, _threads(), _vpp(out)))
+    return _Owned(out, "tns_plans_free")
+
+
+def plan_contract(a_handles, b_handles, pairs):
+    """One contract plan per item pair, planned on host threads."""
+    a_handles = np.ascontiguousarray(a_handles, dtype=np.uint64)
+    b_handles = np.ascontiguousarray(b_handles, dtype=np.uint64)
+    if len(a_handles) != len(b_handles):
+        raise ValueError("batches differ in length")
+    flat = [i for x, y in pairs for i in NAMES.ids([x, y])]
+    return _plans(len(a_handles), _lib.lib().tns_plan_contract_many, _vpp(a_handles), _vpp(b_handles), len(pairs),
+                  _lib.arr32(flat))
+
+
+def plan_qr(handles, direction, free_names, cq, cr):
+    handles = np.ascontiguousarray(handles, dtype=np.uint64)
+    return _plans(len(handles), _lib.lib().tns_plan_qr_many, _vpp(handles), direction.encode()[:1], len(free_names),
+                  _lib.arr32(NAMES.ids(free_names)), NAMES.id(cq), NAMES.id(cr))
+
+
+def plan_svd(handles, free_u, cu, cv, su, sv):
+    handles = np.ascontiguousarray(handles, dtype=np.uint64)
+    return _plans(len(handles), _lib.lib().tns_plan_svd_many, _vpp(handles), len(free_u), _lib.arr32(NAMES.ids(free_u)),
+                  NAMES.id(cu), NAMES.id(cv), NAMES.id(su), NAMES.id(sv))
+
+
+def _names_of(handle, kind):
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().tns_shape_rename(ctypes.c_void_p(int(handle)), 0, _lib.arr32([]), _lib.arr32([]),
+                                           ctypes.byref(h)))
+    return Shape(h, kind).names
+
+
 def _outputs(plans, which, model, dtype):
-    """Output batches ``which`` of a list of plans: one device buffer, one
-    view per item."""
+    """Output batches ``which`` of a plan list: one device buffer per output,
+    items packed back to back."""
     torch = _torch()
+    n = len(plans.arr)
+    L = _lib.lib()
     res = []
     for w in which:
-        hs = []
-        for p in plans:
-            h = ctypes.c_void_p()
-            _lib.check(_lib.lib().tns_plan_output_shape(p.handle, w, ctypes.byref(h)))
-            hs.append(h)
-        shapes = [Shape(h, model.kind, lazy=True) for h in hs]
-        names = shapes[0].names
-        sizes = [s.size for s in shapes]
-        buf = torch.empty(max(1, sum(sizes)), dtype=_tdtype(dtype), device=device())
-        data, off = [], 0
-        for n in sizes:
-            data.append(buf[off:off + n])
-            off += n
-        res.append(TensorBatch(model, dtype, names, shapes, data))
+        hs = np.zeros(n, dtype=np.uint64)
+        sizes = np.zeros(n, dtype=np.int64)
+        _lib.check(L.tns_plans_output_shapes(n, _vpp(plans.arr), w, _vpp(hs), _i64p(sizes)))
+        owner = _Owned(hs, "tns_shapes_free")
+        offs = np.zeros(n, dtype=np.int64)
+        np.cumsum(sizes[:-1], out=offs[1:])
+        buf = torch.empty(max(1, int(sizes.sum())), dtype=_tdtype(dtype), device=device())
+        res.append(TensorBatch(model, dtype, _names_of(hs[0], model.kind), hs, sizes, [buf], np.zeros(n), offs,
+                               owner))
     return res
 
 
 def _workspaces(plans, dtype):
-    torch = _torch()
-    sizes = [p.workspace(dtype) for p in plans]
-    offs, tot = [], 0
-    for s in sizes:
-        offs.append(tot)
-        tot += (s + 255) // 256 * 256
-    buf = torch.empty(max(256, tot), dtype=torch.uint8, device=device())
-    base = buf.data_ptr()
-    return buf, [base + o for o in offs]
+    n = len(plans.arr)
+    sizes = np.zeros(n, dtype=np.int64)
+    _lib.check(_lib.lib().tns_plans_workspace(n, _vpp(plans.arr), DTYPE_CODE[dtype], _i64p(sizes)))
+    padded = (sizes + 255) // 256 * 256
+    offs = np.zeros(n, dtype=np.uint64)
+    np.cumsum(padded[:-1], out=offs[1:])
+    buf = _torch().empty(max(256, int(padded.sum())), dtype=_torch().uint8, device=device())
+    return buf, np.ascontiguousarray(offs + np.uint64(buf.data_ptr()))
 
 
 def _run(plans, dtype, inputs, outputs, workspaces):
-    _lib.check(_lib.lib().tns_plans_run(len(plans), _vp_array([p.handle.value for p in plans]), DTYPE_CODE[dtype],
-                                        _vp_array(inputs), _vp_array(outputs), _vp_array(workspaces),
+    _lib.check(_lib.lib().tns_plans_run(len(plans.arr), _vpp(plans.arr), DTYPE_CODE[dtype],
+                                        _vpp(np.ascontiguousarray(inputs, dtype=np.uint64)),
+                                        _vpp(np.ascontiguousarray(outputs, dtype=np.uint64)), _vpp(workspaces),
                                         ctypes.c_void_p(_stream())))
 
 
-def _plans(handles, type_, kind):
-    return [Plan(ctypes.c_void_p(h), type_, kind) for h in handles]
+def _interleave(*cols):
+    return np.stack(cols, axis=1).ravel()
 
 
-def _handles(shapes):
-    return _vp_array([s.handle.value for s in shapes])
-
-
-# -- host planning (no device needed) -------------------------------------------
-def plan_contract(shapes_a, shapes_b, pairs):
-    """One contract plan per item pair, planned on host threads."""
-    n = len(shapes_a)
-    if len(shapes_b) != n:
-        raise ValueError("batches differ in length")
-    flat = [i for x, y in pairs for i in NAMES.ids([x, y])]
-    out = (ctypes.c_void_p * max(1, n))()
-    _lib.check(_lib.lib().tns_plan_contract_many(n, _handles(shapes_a), _handles(shapes_b), len(pairs),
-                                                 _lib.arr32(flat), _threads(), out))
-    return _plans(out[:n], "contract", shapes_a[0].kind)
-
-
-def plan_qr(shapes, direction, free_names, cq, cr):
-    n = len(shapes)
-    out = (ctypes.c_void_p * max(1, n))()
-    _lib.check(_lib.lib().tns_plan_qr_many(n, _handles(shapes), direction.encode()[:1], len(free_names),
-                                           _lib.arr32(NAMES.ids(free_names)), NAMES.id(cq), NAMES.id(cr),
-                                           _threads(), out))
-    return _plans(out[:n], "qr", shapes[0].kind)
-
-
-def plan_svd(shapes, free_u, cu, cv, su, sv):
-    n = len(shapes)
-    out = (ctypes.c_void_p * max(1, n))()
-    _lib.check(_lib.lib().tns_plan_svd_many(n, _handles(shapes), len(free_u), _lib.arr32(NAMES.ids(free_u)),
-                                            NAMES.id(cu), NAMES.id(cv), NAMES.id(su), NAMES.id(sv), _threads(), out))
-    return _plans(out[:n], "svd", shapes[0].kind)
-
-
+# -- ops ----------------------------------------------------------------------------------
 def contract(a, b, pairs):
     """Item-wise ``a[i].contract(b[i], pairs)``; pairs whose names are absent
     on either side are dropped first (safe_contract,
     tetragono/tetragono/utility.py:340-350)."""
-    n = len(a)
+    if len(a) != len(b):
+        raise ValueError("batches differ in length")
     pr = [(x, y) for x, y in pairs if x in a.names and y in b.names]
-    plans = plan_contract(a.shapes, b.shapes, pr)
+    plans = plan_contract(a.handles, b.handles, pr)
     (res,) = _outputs(plans, [0], a.model, a.dtype)
     ws, wps = _workspaces(plans, a.dtype)
-    ins = [p for i in range(n) for p in (a.data[i].data_ptr(), b.data[i].data_ptr())]
-    _run(plans, a.dtype, ins, [d.data_ptr() for d in res.data], wps)
+    _run(plans, a.dtype, _interleave(a.ptrs, b.ptrs), res.ptrs, wps)
     return res
 
 
 def qr(t, direction, free_names, cq, cr):
-    n = len(t)
-    plans = plan_qr(t.shapes, direction, [x for x in free_names if x in t.names], cq, cr)
+    plans = plan_qr(t.handles, direction, [x for x in free_names if x in t.names], cq, cr)
     q, r = _outputs(plans, [0, 1], t.model, t.dtype)
     ws, wps = _workspaces(plans, t.dtype)
-    outs = [p for i in range(n) for p in (q.data[i].data_ptr(), r.data[i].data_ptr())]
-    _run(plans, t.dtype, [d.data_ptr() for d in t.data], outs, wps)
+    _run(plans, t.dtype, t.ptrs, _interleave(q.ptrs, r.ptrs), wps)
     return q, r
 
 
 def svd(t, free_u, cu, cv, su, sv, cut=-1):
-    n = len(t)
-    plans = plan_svd(t.shapes, [x for x in free_u if x in t.names], cu, cv, su, sv)
+    plans = plan_svd(t.handles, [x for x in free_u if x in t.names], cu, cv, su, sv)
     ws, wps = _workspaces(plans, t.dtype)
     remain, relative = cut_params(cut)
-    _lib.check(_lib.lib().tns_svd_factor_many(n, _vp_array([p.handle.value for p in plans]), DTYPE_CODE[t.dtype],
-                                              _vp_array([d.data_ptr() for d in t.data]), _vp_array(wps), remain,
-                                              relative, _threads(), ctypes.c_void_p(_stream())))
+    n = len(t)
+    _lib.check(_lib.lib().tns_svd_factor_many(n, _vpp(plans.arr), DTYPE_CODE[t.dtype],
+                                              _vpp(np.ascontiguousarray(t.ptrs)), _vpp(wps), remain, relative,
+                                              _threads(), ctypes.c_void_p(_stream())))
     u, s, v = _outputs(plans, [0, 1, 2], t.model, t.dtype)
-    outs = [p for i in range(n) for p in (u.data[i].data_ptr(), s.data[i].data_ptr(), v.data[i].data_ptr())]
-    _run(plans, t.dtype, [d.data_ptr() for d in t.data], outs, wps)
+    _run(plans, t.dtype, t.ptrs, _interleave(u.ptrs, s.ptrs, v.ptrs), wps)
     return u, s, v

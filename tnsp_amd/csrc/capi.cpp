@@ -17,12 +17,14 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/tnsgpu.h"
 #include "core.hpp"
+#include "engine.hpp"
 #include "handles.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
@@ -478,22 +480,28 @@ int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const*
 // The same ops for many tensors at once: planning spread over host threads,
 // execution as one launch per kernel kind for all plans together.
 namespace {
+// One persistent pool serves the item loops (a batch op plans tens to
+// hundreds of items in tens of microseconds each: spawning threads per call
+// would cost more than the work).  A caller that finds the pool busy on
+// another thread plans its items serially.
+std::mutex pool_mu;
+
+Pool& shared_pool(int threads) {
+    static std::unique_ptr<Pool> pool;
+    if (!pool || pool->size() != threads) pool = std::make_unique<Pool>(threads);
+    return *pool;
+}
+
 void parallel_items(int n, int threads, const std::function<void(int)>& fn) {
-    const int T = std::max(1, std::min(threads, n));
-    std::vector<std::exception_ptr> errs(static_cast<size_t>(T));
-    auto work = [&](int k) {
-        try {
-            for (int i = k; i < n; i += T) fn(i);
-        } catch (...) {
-            errs[static_cast<size_t>(k)] = std::current_exception();
-        }
-    };
-    std::vector<std::thread> ts;
-    for (int k = 1; k < T; k++) ts.emplace_back(work, k);
-    work(0);
-    for (auto& t : ts) t.join();
-    for (auto& e : errs)
-        if (e) std::rethrow_exception(e);
+    const int T = std::max(1, std::min(threads, 64));
+    std::unique_lock<std::mutex> lk(pool_mu, std::try_to_lock);
+    if (!lk.owns_lock() || T == 1 || n < 4) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    shared_pool(T).parallel_for(n, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; i++) fn(int(i));
+    });
 }
 
 void check_same_type(int n, tns_plan* const* plans) {
@@ -557,6 +565,63 @@ int tns_plan_svd_many(int n, const tns_shape* const* t, int nfree, const int32_t
             for (int i = 0; i < n; i++) delete out[i];
             throw;
         }
+    })
+}
+
+int tns_shapes_free(int n, tns_shape* const* shapes) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) delete shapes[i];
+    })
+}
+
+int tns_plans_free(int n, tns_plan* const* plans) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) delete plans[i];
+    })
+}
+
+int tns_shape_rename_many(int n, const tns_shape* const* shapes, int nren, const int32_t* old_names,
+                          const int32_t* new_names, tns_shape** out) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            for (int i = 0; i < n; i++) {
+                Shape s = shapes[i]->s;
+                for (int k = 0; k < nren; k++) {
+                    const int f = s.index_of(old_names[k]);
+                    if (f < 0) throw Error("Name missing in edge_rename");
+                    s.names[f] = new_names[k];
+                }
+                for (int a = 0; a < s.rank(); a++)
+                    for (int b = a + 1; b < s.rank(); b++)
+                        if (s.names[a] == s.names[b]) throw Error("duplicated names after edge_rename");
+                out[i] = new_handle(s);
+            }
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
+int tns_plans_output_shapes(int n, tns_plan* const* plans, int which, tns_shape** out, int64_t* sizes) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            for (int i = 0; i < n; i++) {
+                if (tns_plan_output_shape(plans[i], which, &out[i]) != 0) throw Error(tns_last_error());
+                sizes[i] = out[i]->s.size;
+            }
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
+int tns_plans_workspace(int n, tns_plan* const* plans, int dtype, int64_t* bytes) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) bytes[i] = layout_of(plans[i], dtype).total;
     })
 }
 
@@ -678,6 +743,7 @@ int tns_svd_factor_many(int n, tns_plan* const* plans, int dtype, const void* co
         const int64_t es = dtype == 0 ? 8 : 4;
         std::vector<Layout> L(static_cast<size_t>(n));
         Batch bt(dtype);
+        bt.set_fast_svd(true);
         for (int i = 0; i < n; i++) {
             L[size_t(i)] = layout_of(plans[i], dtype);
             char* ws = static_cast<char*>(workspaces[i]);
@@ -720,6 +786,7 @@ int tns_svd_factor(tns_plan* p, int dtype, const void* input, void* workspace, i
         char* ws = static_cast<char*>(workspace);
         void* bufs[5] = {const_cast<void*>(input), ws + L.off[1], ws + L.off[2], ws + L.off[3], ws + L.off[4]};
         Batch bt(dtype);
+        bt.set_fast_svd(true);
         bt.perm(p->s.perm_in, bufs, es);
         bt.flush_perm();
         bt.svd(p->s.svds, bufs, es, reinterpret_cast<double*>(ws + L.scratch));

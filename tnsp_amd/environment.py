@@ -72,11 +72,7 @@ class EnvironmentSampler:
     # -- batches of site tensors --------------------------------------------------
     def _site(self, l1, l2, occ):
         """Projected site (l1, l2) for occupations ``occ`` (one per item)."""
-        pair = self._proj[l1][l2]
-        t0 = pair[0]
-        shapes = [pair[o]._shape for o in occ]
-        data = [pair[o]._data for o in occ]
-        return Bt.TensorBatch(t0.model, t0.dtype, t0.names, shapes, data)
+        return Bt.TensorBatch.select(self._proj[l1][l2], occ)
 
     def _row(self, l1):
         return [self._site(l1, l2, self.configs[:, l1, l2]) for l2 in range(self.L2)]
@@ -121,7 +117,10 @@ class EnvironmentSampler:
             wss = np.abs(replace(hop, t0, t1).values())
             self.replaced += len(hop)
             self.proposals += len(hop)
-            acc = u[hop] < (wss / self.psi[hop]) ** (2 * self.alpha)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                # a carried psi of zero (every sector cut away) accepts any
+                # non-zero proposal, as the tensor ratio of the reference does
+                acc = u[hop] < (wss / self.psi[hop]) ** (2 * self.alpha)
             take = hop[acc]
             self.configs[take, a[0], a[1]], self.configs[take, b[0], b[1]] = nb[take], na[take]
             self.psi[take] = wss[acc]
@@ -129,7 +128,8 @@ class EnvironmentSampler:
         if len(dbl) and self.diagonal == "hole":
             wss = np.abs(self.engine.evaluate(self.configs[dbl], cut=self.cut))
             self.refreshed += len(dbl)
-            acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
             self.psi[dbl[acc]] = wss[acc]
 
     # -- passes -----------------------------------------------------------------------
