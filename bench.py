@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile per size class")
     ap.add_argument("--ess", default="", help="also measure ESS/sec of plain vs learned sampling on this workload")
     ap.add_argument("--ess-chains", type=int, default=256)
+    ap.add_argument("--ess-diagonal", default="keep", choices=["keep", "hole"],
+                    help="doubly occupied bonds: keep psi, or refresh it with hole(()) as tetragono does")
+    ap.add_argument("--ess-engine-sweep", action="store_true",
+                    help="also measure the sweep that recomputes psi(s') with the amplitude engine")
     ap.add_argument("--ess-samples", type=int, default=16)
     ap.add_argument("--ess-train-sweeps", type=int, default=6)
     ap.add_argument("--ess-epochs", type=int, default=6)
@@ -128,51 +132,76 @@ def cpu_baseline(spec, storages, configs, per_proc, procs):
 
 
 def ess_comparison(args, log):
-    """Plain sweep sampling vs learned-proposal Metropolis-Hastings on the
-    tV model (t = V = 1): ESS of the energy estimator per second of sampling.
-    The learned proposal is trained first on hops the plain sampler tried;
-    one learned sample is (bonds / 2) proposals per chain, about the
-    amplitude count of one plain sweep."""
+    """Sweep sampling vs learned-proposal Metropolis-Hastings on the tV model
+    (t = V = 1): ESS of the energy estimator per second of sampling.  The
+    sweep is tetragono's: cached-environment ``replace`` for every bond
+    (EnvironmentSampler; ``--ess-diagonal keep`` skips the hole(()) refresh of
+    doubly occupied bonds); ``--ess-engine-sweep`` also times the sweep that
+    recomputes psi(s') with the amplitude engine.  The learned proposal is
+    trained first on hops the sweep tried; one learned sample is K proposals
+    per chain.  Local energies (not timed) are tetragono's observer: every
+    term's psi(s') by replace over cached environments."""
     import numpy as np
 
     from tnsp_amd import peps
     from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
     from tnsp_amd.proposal import HopProposalNet, LearnedSampler, collect_training_data, train
     from tnsp_amd.sampler import TVSampler, measure_ess
 
     spec = getattr(peps, args.ess)
     lat = peps.Lattice(spec)
-    eng = AmplitudeEngine(lat, chunk=1024, lanes=args.lanes)
+    # 256 configurations per lane chunk: at C4 the arenas of one chunk hold
+    # about 30 GB, so four lanes of 1024 would not fit in HBM
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
     L = spec.L1 * spec.L2
     cand = peps.sample_configurations(spec, 4 * args.ess_chains, seed=77)
     psi = eng.evaluate(cand)
     start = cand[psi != 0][:args.ess_chains]
-    plain = TVSampler(eng, start, seed=1)
+    sweep = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal)
+
+    def energy(configs):
+        # tetragono's observer: psi(s') by replace over cached environments
+        return sweep.local_energy(configs)
+
     t0 = time.time()
-    occs, hops, targets = collect_training_data(plain, args.ess_train_sweeps, log=log)
+    occs, hops, targets = collect_training_data(sweep, args.ess_train_sweeps, log=log)
     t_collect = time.time() - t0
     net = HopProposalNet(spec.L1, spec.L2, channels=32, layers=4, seed=3)
     t0 = time.time()
     losses = train(net, occs, hops, targets, epochs=args.ess_epochs, batch=1024, lr=2e-3, log=log)
     t_train = time.time() - t0
-    res_plain = measure_ess(plain.sweep, plain.local_energy, args.ess_samples, sites=L,
-                            log=lambda m: log(f"plain sweep: {m}"))
-    res_plain["acceptance"] = plain.accepted / max(1, plain.proposals)
-    ks = [int(x) for x in args.ess_learned_k.split(",") if x] or [max(1, len(plain.bonds) // 2)]
+    res = {}
+    res_sweep = measure_ess(sweep.sweep, lambda: energy(sweep.configs), args.ess_samples, sites=L,
+                            log=lambda m: log(f"environment sweep: {m}"))
+    res_sweep["acceptance"] = sweep.accepted / max(1, sweep.proposals)
+    res_sweep["diagonal"] = args.ess_diagonal
+    res["sweep"] = res_sweep
+    if args.ess_engine_sweep:
+        plain = TVSampler(eng, sweep.configs.copy(), seed=1)
+        r = measure_ess(plain.sweep, lambda: energy(plain.configs), args.ess_samples, sites=L,
+                        log=lambda m: log(f"engine sweep: {m}"))
+        r["acceptance"] = plain.accepted / max(1, plain.proposals)
+        res["engine_sweep"] = r
+    ks = [int(x) for x in args.ess_learned_k.split(",") if x] or [max(1, len(sweep.bonds) // 2)]
     res_learned = []
     for K in ks:
-        learned = LearnedSampler(eng, net, plain.configs.copy(), seed=2)
+        learned = LearnedSampler(eng, net, sweep.configs.copy(), seed=2)
         r = measure_ess(lambda: [learned.step() for _ in range(K)],
-                        lambda: plain.local_energy(learned.configs, learned.psi), args.ess_samples, sites=L,
+                        lambda: energy(learned.configs), args.ess_samples, sites=L,
                         log=lambda m: log(f"learned K={K}: {m}"))
         r["acceptance"] = learned.accepted / max(1, learned.proposals)
         r["proposals_per_sample"] = K
         res_learned.append(r)
     best = max(res_learned, key=lambda r: r["ess_per_s"])
-    return {"workload": args.ess, "chains": len(start), "plain": res_plain, "learned": res_learned,
-            "gain": best["ess_per_s"] / max(res_plain["ess_per_s"], 1e-12),
-            "training": {"samples": int(len(hops)), "collect_s": t_collect, "train_s": t_train,
-                         "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}}
+    out = {"workload": args.ess, "chains": len(start), "learned": res_learned,
+           "gain": best["ess_per_s"] / max(res_sweep["ess_per_s"], 1e-12),
+           "training": {"samples": int(len(hops)), "collect_s": t_collect, "train_s": t_train,
+                        "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}}
+    out.update(res)
+    if "engine_sweep" in res:
+        out["gain_vs_engine_sweep"] = best["ess_per_s"] / max(res["engine_sweep"]["ess_per_s"], 1e-12)
+    return out
 
 
 def main():

@@ -225,3 +225,26 @@ def sweep(sites, config, cut, uniforms, alpha=1.0, t=1.0, V=1.0):
             if new is not None:
                 cfg[a], cfg[b] = new
     return cfg, trace, abs(ws)
+
+
+def local_energy(sites, config, cut, H):
+    """tetragono's observer energy (sampling_lattice/observer.py:313-389) for
+    one configuration: ws = hole(()); each term's wss from ``replace`` (for
+    a target equal to the configuration, ``hole(())``).  Returns None when
+    ws is zero (the observer skips the configuration)."""
+    from .peps import psi
+    from .sampling import bond_order, element_pool, local_energy_term
+
+    cfg = np.array(config, dtype=np.int64)
+    if psi(sites, cfg, cut)[0] == 0:
+        return None
+    pool = element_pool(H)
+    total = 0.0
+    for pos in bond_order(len(cfg), len(cfg[0])):
+        def wss_of(cfg2, occ_s, pos=pos):
+            if all(int(cfg2[p]) == int(cfg[p]) for p in pos):
+                return Environments(project(sites, cfg), cut).hole()
+            return replace(sites, cfg, cut, pos, occ_s)
+
+        total += local_energy_term(sites, cfg, cut, pos, H, pool, wss_of=wss_of)
+    return total

@@ -99,10 +99,12 @@ def bond_order(L1, L2):
     return order
 
 
-def local_energy_term(sites, config, cut, positions, H, pool=None):
+def local_energy_term(sites, config, cut, positions, H, pool=None, wss_of=None):
     """observer.py:350-379 for one bond: sum over s' of
     ``inv_ws_conj . <s'|H|s> . conj(wss)`` contracted over T and every
-    physical name.  ``positions`` = ((l1, l2), (l1', l2'))."""
+    physical name.  ``positions`` = ((l1, l2), (l1', l2')).  ``wss_of(cfg2,
+    occupations)`` gives <s'|psi> (default: the amplitude of s' from
+    scratch; tetragono's observer uses ``Configuration.replace``)."""
     pool = element_pool(H) if pool is None else pool
     L1, L2 = len(sites), len(sites[0])
     occ = tuple(int(config[p[0]][p[1]]) for p in positions)
@@ -118,7 +120,7 @@ def local_energy_term(sites, config, cut, positions, H, pool=None):
         cfg2 = np.array(config, copy=True)
         for i, p in enumerate(positions):
             cfg2[p[0]][p[1]] = occ_s[i]
-        _, wss = peps.psi(sites, cfg2, cut)
+        wss = peps.psi(sites, cfg2, cut)[1] if wss_of is None else wss_of(cfg2, occ_s)
         if wss.size == 0 or float(np.max(np.abs(wss.data))) == 0:
             continue
         v = inv_ws_conj.contract(elem, {(physics_names[i], f"I{i}") for i in range(len(positions))})

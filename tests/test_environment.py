@@ -121,3 +121,17 @@ def test_sweep_uncut_follows_exact_metropolis():
             cur, psi = new, p_new
     assert np.array_equal(final, cur)
     assert abs(last - psi) <= 1e-10 * psi
+
+
+def test_observer_energy_uncut_equals_scratch_energy():
+    """The observer through replace equals the observer with every psi(s')
+    from scratch when nothing is truncated (3x3, two configurations)."""
+    from oracle import environment as E, sampling
+
+    L1, L2, N = 3, 3, 4
+    _, _, sites = small_lattice(L1, L2, N, seed=12)
+    H = sampling.tv_hamiltonian(1.0, 1.0)
+    for cfg in configs(L1, L2, N, 2, seed=13):
+        got = E.local_energy(sites, cfg, 10_000, H)
+        want = sampling.local_energy(sites, cfg, 10_000, H)
+        assert abs(got - want) <= 1e-10 * max(1.0, abs(want))

@@ -14,7 +14,7 @@ from test_sampler import small_lattice
 pytestmark = pytest.mark.gpu
 
 
-def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1):
+def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1, lanes=1):
     from oracle import environment as E
     from tnsp_amd import peps as pp
     from tnsp_amd.engine import AmplitudeEngine
@@ -23,7 +23,7 @@ def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1):
     spec, storages, sites = small_lattice(L1, L2, N, seed=seed)
     eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
     start = np.stack(configs(L1, L2, N, chains, seed=seed + 1))
-    smp = EnvironmentSampler(eng, start, seed=seed + 2, cut=cut, diagonal=diagonal)
+    smp = EnvironmentSampler(eng, start, seed=seed + 2, cut=cut, diagonal=diagonal, lanes=lanes)
     gen = np.random.Generator(np.random.PCG64(seed + 2))
     nb = len(smp.bonds)
     cur = start.astype(np.int64)
@@ -48,8 +48,52 @@ def test_environment_sweep_matches_oracle_truncated():
     assert smp.accepted > 0 and smp.refreshed > 0
 
 
+def test_environment_sweep_lanes_match_oracle():
+    """Chains split over two lanes (own host thread and stream each) take
+    the same decisions as the oracle, doubly occupied bonds refreshed."""
+    smp = _run(4, 4, 8, 4, chains=3, seed=20, diagonal="hole", lanes=2)
+    assert smp.accepted > 0 and smp.refreshed > 0
+
+
 def test_environment_sweep_two_sweeps_uncut():
     """3x4, two sweeps without truncation; diagonal bonds skipped (the oracle
     with V = 0 has no diagonal element either)."""
     smp = _run(3, 4, 5, 10_000, chains=3, seed=30, diagonal="keep", sweeps=2)
     assert smp.accepted > 0
+
+
+def test_environment_observer_energy_matches_oracle_truncated():
+    """Local energy through replace over cached environments (tetragono's
+    observer) against the oracle observer, 4x4 at a truncating cut."""
+    from oracle import environment as E, sampling
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    L1, L2, N, cut = 4, 4, 8, 4
+    spec, storages, sites = small_lattice(L1, L2, N, seed=40)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    start = np.stack(configs(L1, L2, N, 3, seed=41))
+    smp = EnvironmentSampler(eng, start, cut=cut)
+    got = smp.local_energy()
+    H = sampling.tv_hamiltonian(1.0, 1.0)
+    want = [E.local_energy(sites, c, cut, H) for c in start]
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+    smp.lanes = 2
+    np.testing.assert_allclose(smp.local_energy(), want, rtol=1e-9, atol=1e-9)
+
+
+def test_environment_observer_energy_uncut_equals_engine_energy():
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+    from tnsp_amd.sampler import TVSampler
+
+    L1, L2, N = 3, 4, 5
+    spec, storages, _ = small_lattice(L1, L2, N, seed=42)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    start = np.stack(configs(L1, L2, N, 8, seed=43))
+    smp = EnvironmentSampler(eng, start, cut=10_000)
+    got = smp.local_energy()
+    want = TVSampler(eng, start).local_energy()
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)

@@ -504,6 +504,29 @@ void parallel_items(int n, int threads, const std::function<void(int)>& fn) {
     });
 }
 
+// Host-mapped pinned buffer of this host thread (kernels write into it
+// directly); kept until the thread ends.
+struct MappedHost {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~MappedHost() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+double* mapped_host_buffer(size_t bytes) {
+    thread_local MappedHost buf;
+    if (bytes > buf.cap) {
+        if (buf.p) check_hip(hipHostFree(buf.p), "mapped host free");
+        buf.p = nullptr;
+        buf.cap = 0;
+        const size_t cap = std::max<size_t>(bytes * 2, size_t(1) << 20);
+        check_hip(hipHostMalloc(&buf.p, cap, hipHostMallocMapped | hipHostMallocCoherent), "mapped host alloc");
+        buf.cap = cap;
+    }
+    return static_cast<double*>(buf.p);
+}
+
 void check_same_type(int n, tns_plan* const* plans) {
     for (int i = 1; i < n; i++)
         if (plans[i]->type != plans[0]->type) throw Error("a plan batch must hold one op type");
@@ -761,18 +784,31 @@ int tns_svd_factor_many(int n, tns_plan* const* plans, int dtype, const void* co
         bt.flush_svd();
         hipStream_t st = as_stream(stream);
         bt.run(st);
+        // every item's singular values in one launch, written straight into
+        // host-mapped pinned memory (one per host thread, grown on demand)
+        int64_t total = 0;
+        std::vector<int64_t> soff(static_cast<size_t>(n));
         for (int i = 0; i < n; i++) {
-            auto* p = plans[i];
-            p->svals_host.assign(size_t(p->s.s_total), 0.0);
-            if (p->s.s_total)
-                check_hip(hipMemcpyAsync(p->svals_host.data(), static_cast<char*>(workspaces[i]) + L[size_t(i)].off[4],
-                                         size_t(p->s.s_total) * 8, hipMemcpyDeviceToHost, st),
-                          "singular values copy");
+            soff[size_t(i)] = total;
+            total += plans[i]->s.s_total;
         }
+        double* host = mapped_host_buffer(size_t(n) * sizeof(GatherDev) + size_t(total) * 8);
+        auto* tasks = reinterpret_cast<GatherDev*>(host);
+        double* vals = host + size_t(n) * sizeof(GatherDev) / 8;
+        for (int i = 0; i < n; i++)
+            tasks[i] = GatherDev{reinterpret_cast<const double*>(static_cast<char*>(workspaces[i]) + L[size_t(i)].off[4]),
+                                 plans[i]->s.s_total, soff[size_t(i)]};
+        GatherDev* dtasks = nullptr;
+        double* dvals = nullptr;
+        check_hip(hipHostGetDevicePointer(reinterpret_cast<void**>(&dtasks), tasks, 0), "mapped pointer");
+        dvals = reinterpret_cast<double*>(dtasks) + size_t(n) * sizeof(GatherDev) / 8;
+        launch_gather_f64(dtasks, n, dvals, st);
         check_hip(hipStreamSynchronize(st), "svd synchronize");
         parallel_items(n, threads, [&](int i) {
-            plan_svd_truncate(plans[i]->s, plans[i]->svals_host.data(), remain_cut, relative_cut);
-            plans[i]->factored = true;
+            auto* p = plans[i];
+            p->svals_host.assign(vals + soff[size_t(i)], vals + soff[size_t(i)] + p->s.s_total);
+            plan_svd_truncate(p->s, p->svals_host.data(), remain_cut, relative_cut);
+            p->factored = true;
         });
     })
 }

@@ -1193,6 +1193,19 @@ void launch_fill(const FillDev* tasks, const int64_t* prefix, const int32_t* wg_
     check(hipGetLastError(), "fill launch");
 }
 
+namespace {
+__global__ void __launch_bounds__(256) gather_kernel(const GatherDev* __restrict__ tasks, double* __restrict__ dst) {
+    const GatherDev t = tasks[blockIdx.x];
+    for (int64_t i = threadIdx.x; i < t.n; i += blockDim.x) dst[t.off + i] = t.src[i];
+}
+}  // namespace
+
+void launch_gather_f64(const GatherDev* tasks, int64_t num, double* dst, hipStream_t stream) {
+    if (num <= 0) return;
+    hipLaunchKernelGGL(gather_kernel, dim3(unsigned(num)), dim3(256), 0, stream, tasks, dst);
+    check(hipGetLastError(), "gather launch");
+}
+
 void launch_permute_to_f64(const PermDev* tasks, const int64_t* prefix, const int32_t* wg_first, int64_t num_wg,
                            hipStream_t stream) {
     if (num_wg <= 0) return;

@@ -105,6 +105,16 @@ constexpr int64_t QB_MAX_ROWS = (QB_LDS - 8 * (16 * 16 * 2 + 16 * 128 + 16)) / (
 // (rpt in {8, 16, 24}), V recovered from U; otherwise LDS (lds_bytes > 0)
 // or global scratch.
 void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, int dtype, hipStream_t stream);
+
+// Copy n_i doubles from src_i to dst + off_i for every task, one workgroup
+// per task: gathers the singular values of a batch of decompositions into
+// one (host-mapped) buffer with a single launch.
+struct GatherDev {
+    const double* src;
+    int64_t n;
+    int64_t off;
+};
+void launch_gather_f64(const GatherDev* tasks, int64_t num, double* dst, hipStream_t stream);
 constexpr int SVD_REG_MAX_ROWS = 192;
 
 }  // namespace tns

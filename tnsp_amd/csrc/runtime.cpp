@@ -12,21 +12,26 @@ void check_hip(hipError_t e, const char* what) {
 }
 
 Staging& Staging::get() {
-    static Staging s;
+    // per host thread: callers on different threads (and streams) never share
+    // the buffers
+    thread_local Staging s;
     return s;
 }
 
 Staging::~Staging() {
-    if (pending_ && ev_) hipEventSynchronize(ev_);
-    if (ev_) hipEventDestroy(ev_);
-    if (host_) hipHostFree(host_);
-    if (dev_) hipFree(dev_);
+    if (pending_ && ev_) (void)hipEventSynchronize(ev_);
+    if (in_use_ && done_) (void)hipEventSynchronize(done_);
+    if (ev_) (void)hipEventDestroy(ev_);
+    if (done_) (void)hipEventDestroy(done_);
+    if (host_) (void)hipHostFree(host_);
+    if (dev_) (void)hipFree(dev_);
 }
 
 char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
     const size_t n = blob.size();
     if (n == 0) return dev_;
     if (!ev_) check_hip(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "event create");
+    if (!done_) check_hip(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "event create");
     // the pinned staging buffer may still feed the previous copy
     if (pending_) {
         check_hip(hipEventSynchronize(ev_), "staging wait");
@@ -38,18 +43,29 @@ char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
         check_hip(hipHostMalloc(reinterpret_cast<void**>(&host_), host_cap_, hipHostMallocDefault), "host alloc");
     }
     if (n > dev_cap_) {
-        // kernels queued earlier may still read the old task buffer; the
-        // process-wide instance may have fed more than one stream
-        check_hip(this == &Staging::get() ? hipDeviceSynchronize() : hipStreamSynchronize(st), "staging synchronize");
+        // kernels queued earlier may still read the old task buffer
+        if (in_use_) check_hip(hipEventSynchronize(done_), "staging synchronize");
+        in_use_ = false;
         if (dev_) check_hip(hipFree(dev_), "device free");
         dev_cap_ = std::max(n, dev_cap_ * 2);
         check_hip(hipMalloc(reinterpret_cast<void**>(&dev_), dev_cap_), "device alloc");
     }
+    // kernels of the previous upload, if queued on another stream, finish
+    // before this copy overwrites their records (same stream: in order)
+    if (in_use_ && last_ != st) check_hip(hipStreamWaitEvent(st, done_, 0), "staging stream wait");
     std::memcpy(host_, blob.data(), n);
     check_hip(hipMemcpyAsync(dev_, host_, n, hipMemcpyHostToDevice, st), "task upload");
     check_hip(hipEventRecord(ev_, st), "event record");
     pending_ = true;
+    last_ = st;
     return dev_;
+}
+
+void Staging::release(hipStream_t st) {
+    if (!done_) return;
+    check_hip(hipEventRecord(done_, st), "staging release");
+    in_use_ = true;
+    last_ = st;
 }
 
 hipEvent_t KernelTimer::get() {
@@ -370,7 +386,18 @@ void Batch::run(hipStream_t st) {
     flush_qr();
     flush_svd();
     if (cmds_.empty()) return;
-    char* base = (staging_ ? *staging_ : Staging::get()).upload(blob_, st);
+    Staging& stg = staging_ ? *staging_ : Staging::get();
+    char* base = stg.upload(blob_, st);
+    struct Release {
+        Staging& s;
+        hipStream_t st;
+        ~Release() {
+            try {
+                s.release(st);
+            } catch (...) {
+            }
+        }
+    } release{stg, st};
     for (auto& c : cmds_) {
         hipEvent_t ea = nullptr, eb = nullptr;
         if (timer_) {

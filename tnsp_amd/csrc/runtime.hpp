@@ -19,6 +19,11 @@ void check_hip(hipError_t e, const char* what);
 // Device/pinned staging for task arrays.  Not thread-safe: one user (one
 // stream) at a time; the per-op API shares the process-wide instance, each
 // engine lane owns its own.
+// Task records of a batch travel host -> pinned buffer -> device buffer on
+// the launch stream.  One instance per host thread (get()) or per engine
+// lane.  The device buffer is reused by the next upload, possibly on another
+// stream: release(st), recorded after the launches that read it, makes that
+// upload wait for them on the device.
 class Staging {
    public:
     static Staging& get();
@@ -28,14 +33,19 @@ class Staging {
     ~Staging();
     // copies blob to device memory on stream st and returns the device base
     char* upload(const std::vector<char>& blob, hipStream_t st);
+    // the kernels reading the last upload are queued on st
+    void release(hipStream_t st);
 
    private:
     char* dev_ = nullptr;
     size_t dev_cap_ = 0;
     char* host_ = nullptr;
     size_t host_cap_ = 0;
-    hipEvent_t ev_ = nullptr;
+    hipEvent_t ev_ = nullptr;    // copy out of host_ done
     bool pending_ = false;
+    hipEvent_t done_ = nullptr;  // kernels reading dev_ done
+    bool in_use_ = false;
+    hipStream_t last_ = nullptr;
 };
 
 // Per-kind device timing of batched launches with HIP events recorded on the

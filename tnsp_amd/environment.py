@@ -32,6 +32,9 @@ amplitude computed afresh, and the draw compares it with the carried psi.
 boundary-MPS truncation, so the step accepts with probability one up to
 that difference).
 """
+import threading
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import batch as Bt
@@ -43,9 +46,10 @@ class EnvironmentSampler:
     """Lockstep sweep sampling of B chains of a FermiU1 or FermiZ2 PEPS with
     the tV Hamiltonian (hops t, density interaction V)."""
 
-    def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole"):
+    def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole", lanes=1):
         if diagonal not in ("hole", "keep"):
             raise ValueError('diagonal must be "hole" or "keep"')
+        self.lanes = max(1, int(lanes))
         self.engine = engine
         self.lattice = engine.lattice
         spec = engine.spec
@@ -58,6 +62,7 @@ class EnvironmentSampler:
             raise ValueError("configs must have shape (chains, L1, L2)")
         self.gen = np.random.Generator(np.random.PCG64(seed))
         self.bonds = bond_order(self.L1, self.L2)
+        self._bond_index = {bd: k for k, bd in enumerate(self.bonds)}
         self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
                        for l2 in range(self.L2)] for l1 in range(self.L1)]
         self._one = _one(spec, self.lattice.dtype)
@@ -68,6 +73,17 @@ class EnvironmentSampler:
         self.accepted = 0
         self.replaced = 0
         self.refreshed = 0
+        # called as recorder(a, b, before, psi_old, psi_new) for every bond
+        # with proposals: flat configurations before the hop and |psi|
+        self.recorder = None
+        self._lock = threading.Lock()
+        self._u = None
+        self._pool = None
+        self._streams = []
+        from .sampler import _fermion_order_index, hole_name_order
+
+        self._hole_pos = {n: i for i, n in enumerate(hole_name_order(self.L1, self.L2))}
+        self._site_pos, self._t_pos = _fermion_order_index(self.L1, self.L2)
 
     # -- batches of site tensors --------------------------------------------------
     def _site(self, l1, l2, occ):
@@ -105,16 +121,18 @@ class EnvironmentSampler:
         """``_single_term`` on bond (a, b) for every chain: one uniform per
         chain; chains with one particle on the bond propose the hop and read
         psi(s') from ``replace(idx, t0, t1)``."""
-        B = len(self.configs)
         na = self.configs[:, a[0], a[1]].copy()
         nb = self.configs[:, b[0], b[1]].copy()
-        u = self.gen.random(B)
+        u = self._u[self._bond_index[(a, b)]]
         hop = np.nonzero(na != nb)[0]
         dbl = np.nonzero((na == 1) & (nb == 1))[0] if self.V != 0 else np.zeros(0, dtype=np.int64)
         if len(hop):
             t0 = self._site(a[0], a[1], nb[hop])
             t1 = self._site(b[0], b[1], na[hop])
             wss = np.abs(replace(hop, t0, t1).values())
+            if self.recorder is not None:
+                with self._lock:
+                    self.recorder(a, b, self.configs[hop].reshape(len(hop), -1).copy(), self.psi[hop].copy(), wss)
             self.replaced += len(hop)
             self.proposals += len(hop)
             with np.errstate(divide="ignore", invalid="ignore"):
@@ -126,14 +144,15 @@ class EnvironmentSampler:
             self.psi[take] = wss[acc]
             self.accepted += int(acc.sum())
         if len(dbl) and self.diagonal == "hole":
-            wss = np.abs(self.engine.evaluate(self.configs[dbl], cut=self.cut))
+            with self._lock:  # one engine handle: one caller at a time
+                wss = np.abs(self.engine.evaluate(self.configs[dbl], cut=self.cut))
             self.refreshed += len(dbl)
             with np.errstate(divide="ignore", invalid="ignore"):
                 acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
             self.psi[dbl[acc]] = wss[acc]
 
     # -- passes -----------------------------------------------------------------------
-    def _horizontal(self, log=None):
+    def _horizontal(self, log=None, observe=False):
         L1, L2 = self.L1, self.L2
         B = len(self.configs)
         one = self._ones(B)
@@ -156,7 +175,10 @@ class EnvironmentSampler:
                 rt[c] = tailed
             if l1 == 0:
                 # hole(()) = _inline_right_to_left[0, 0]: psi at the start of the sweep
-                self.psi = np.abs(rt[0].values())
+                if observe:
+                    self._hole = rt[0].values()
+                else:
+                    self.psi = np.abs(rt[0].values())
             # _inline_left_to_right_tailed[l1, -1] (:254-282)
             lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
             for l2 in range(L2 - 1):
@@ -169,7 +191,7 @@ class EnvironmentSampler:
                     return Bt.contract(res, rt[l2 + 2].take(idx), [("R1", "L1"), ("R2", "L2"), ("D", "U"),
                                                                     ("R3", "L3")])
 
-                self._term((l1, l2), (l1, l2 + 1), replace)
+                (self._observe if observe else self._term)((l1, l2), (l1, l2 + 1), replace)
                 site = self._site(l1, l2, self.configs[:, l1, l2])
                 left = Bt.contract(lt, site.rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
                 left = Bt.contract(left, dn[l2].rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
@@ -179,7 +201,7 @@ class EnvironmentSampler:
             if log:
                 log(f"row {l1} done")
 
-    def _vertical(self, log=None):
+    def _vertical(self, log=None, observe=False):
         L1, L2 = self.L1, self.L2
         B = len(self.configs)
         one = self._ones(B)
@@ -212,7 +234,7 @@ class EnvironmentSampler:
                     return Bt.contract(res, dt[l1 + 2].take(idx), [("D1", "U1"), ("D2", "U2"), ("R", "L"),
                                                                     ("D3", "U3")])
 
-                self._term((l1, l2), (l1 + 1, l2), replace)
+                (self._observe if observe else self._term)((l1, l2), (l1 + 1, l2), replace)
                 site = self._site(l1, l2, self.configs[:, l1, l2])
                 inl = Bt.contract(ut, site.rename({"D": "D2"}), [("D2", "U"), ("R", "L")])
                 inl = Bt.contract(inl, rc[l1].rename({"D": "D3"}), [("D3", "U"), ("R", "L")])
@@ -222,11 +244,147 @@ class EnvironmentSampler:
             if log:
                 log(f"column {l2} done")
 
+    # -- local energy (observer.py:313-389) ------------------------------------------
+    def _reorder_sign(self, names, flat):
+        """Sign that brings a one-element tensor with edges ``names`` (all
+        of dimension one: P_{l1}_{l2}_0 and T) to the edge order of
+        ``hole(())``: (-1)^(pairs of odd edges whose order flips), the
+        transpose sign of TAT's edge operator for such tensors.  ``flat``:
+        occupations of the configuration the tensor belongs to."""
+        pos = np.empty(len(names), dtype=np.int64)
+        odd = np.empty((len(flat), len(names)), dtype=np.int64)
+        for j, n in enumerate(names):
+            pos[j] = self._hole_pos[n]
+            if n == "T":
+                odd[:, j] = self.engine.spec.N % 2
+            elif n.startswith("P_"):
+                _, a, b, _ = n.split("_")
+                odd[:, j] = flat[:, int(a) * self.L2 + int(b)]
+            else:
+                raise ValueError(f"replace left a virtual edge {n} uncontracted")
+        flips = np.triu(pos[:, None] > pos[None, :], 1).astype(np.int64)
+        count = np.einsum("ci,ij,cj->c", odd, flips, odd)
+        return 1 - 2 * (count % 2)
+
+    def _observe(self, a, b, replace):
+        """One bond of the energy observer: -t (-1)^k psi(s^ab) / psi(s) for
+        every chain with one particle on the bond (psi(s^ab) from
+        ``replace``, brought to the edge order of ``hole(())``)."""
+        na = self.configs[:, a[0], a[1]]
+        nb = self.configs[:, b[0], b[1]]
+        hop = np.nonzero(na != nb)[0]
+        if len(hop) == 0:
+            return
+        res = replace(hop, self._site(a[0], a[1], nb[hop]), self._site(b[0], b[1], na[hop]))
+        before = self.configs[hop].reshape(len(hop), -1).astype(np.int64)
+        after = before.copy()
+        ia, ib = a[0] * self.L2 + a[1], b[0] * self.L2 + b[1]
+        after[:, [ia, ib]] = after[:, [ib, ia]]
+        psi_new = res.values() * self._reorder_sign(res.names, after)
+        coef = -self.t * self._hop_sign(before, ia, ib)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            self._energy[hop] += coef * psi_new / self._hole[hop]
+
+    def _hop_sign(self, flat, ia, ib):
+        pa, pb = self._site_pos[ia], self._site_pos[ib]
+        lo, hi = min(pa, pb), max(pa, pb)
+        between = (self._site_pos > lo) & (self._site_pos < hi)
+        k = flat[:, between].sum(1)
+        if lo < self._t_pos < hi:
+            k = k + self.engine.spec.N
+        return 1 - 2 * (k % 2)
+
+    def local_energy(self, configs=None, log=None):
+        """E_loc of every chain as tetragono's observer computes it
+        (observer.py:313-389): psi(s) = hole(()), each hop term's psi(s')
+        from ``replace`` over the cached environments, the diagonal term
+        V n_i n_j.  Chains whose psi(s) is exactly zero get NaN (the observer
+        skips such configurations).  ``configs`` defaults to the chains."""
+        cfg = self.configs if configs is None else np.ascontiguousarray(np.asarray(configs), dtype=np.int8)
+        out = np.zeros(len(cfg))
+
+        def run(child, idx):
+            child.configs = cfg[idx].copy()
+            flat = child.configs.reshape(len(idx), -1).astype(np.int64)
+            child._energy = np.zeros(len(idx))
+            for (a, b) in child.bonds:
+                ia, ib = a[0] * child.L2 + a[1], b[0] * child.L2 + b[1]
+                child._energy += child.V * flat[:, ia] * flat[:, ib]
+            child._horizontal(log, observe=True)
+            child._vertical(log, observe=True)
+            e = child._energy
+            e[child._hole == 0] = np.nan
+            out[idx] = e
+
+        self._in_lanes(len(cfg), run)
+        return out
+
     def sweep(self, log=None):
         """One ``SweepSampling.__call__`` for every chain; returns the
         configurations.  ``self.psi`` holds |psi| as carried by the sweep
         (the last accepted ``replace`` value of each chain).  ``log``, if
-        given, is called with a progress line after every row and column."""
-        self._horizontal(log)
-        self._vertical(log)
+        given, is called with a progress line after every row and column.
+        One uniform per chain and bond, drawn for all chains bond by bond,
+        whatever the number of lanes."""
+        self._u = self.gen.random((len(self.bonds), len(self.configs)))
+
+        def run(child, idx):
+            child.configs = self.configs[idx].copy()
+            child.psi = self.psi[idx].copy()
+            child._u = self._u[:, idx]
+            child._horizontal(log)
+            child._vertical(log)
+            self.configs[idx] = child.configs
+            self.psi[idx] = child.psi
+
+        self._in_lanes(len(self.configs), run)
         return self.configs
+
+    # -- lanes --------------------------------------------------------------------------
+    def _child(self):
+        """A view of this sampler for one group of chains: shares the engine,
+        projected sites, lock and recorder; counts its own proposals."""
+        c = object.__new__(EnvironmentSampler)
+        c.__dict__.update(self.__dict__)
+        c.proposals = c.accepted = c.replaced = c.refreshed = 0
+        return c
+
+    def _in_lanes(self, n, run):
+        """run(child, chain indices) for ``lanes`` contiguous groups of the
+        n chains, each on its own host thread and HIP stream (the batched
+        ops of one group overlap the host work and waits of the others);
+        one group runs on the caller's thread and stream."""
+        groups = [g for g in np.array_split(np.arange(n), min(self.lanes, max(1, n))) if len(g)]
+        children = [self._child() for _ in groups]
+        if len(groups) == 1:
+            run(children[0], groups[0])
+        else:
+            import torch
+
+            if self._pool is None or len(self._streams) < len(groups):
+                # persistent workers: the native library keeps per-thread
+                # task staging and pinned buffers
+                self._pool = ThreadPoolExecutor(max_workers=len(groups))
+                self._streams = [torch.cuda.Stream() for _ in groups]
+
+            def work(child, idx, stream):
+                with torch.cuda.stream(stream):
+                    run(child, idx)
+                stream.synchronize()
+
+            main = torch.cuda.current_stream()
+            streams = self._streams[:len(groups)]
+            for st in streams:
+                st.wait_stream(main)
+            futures = [self._pool.submit(work, c, g, st) for c, g, st in zip(children, groups, streams)]
+            errors = [f.exception() for f in futures]
+            for st in streams:
+                main.wait_stream(st)
+            for e in errors:
+                if e is not None:
+                    raise e
+        for c in children:
+            self.proposals += c.proposals
+            self.accepted += c.accepted
+            self.replaced += c.replaced
+            self.refreshed += c.refreshed

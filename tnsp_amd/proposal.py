@@ -222,22 +222,34 @@ class LearnedSampler:
 
 def collect_training_data(sampler, sweeps, max_samples=None, log=None):
     """Runs plain sweeps and records every tried hop: (configuration before,
-    hop index, log|psi(s')/psi(s)|).  ``sampler`` is a TVSampler."""
+    hop index, log|psi(s')/psi(s)|).  ``sampler`` is a TVSampler or an
+    EnvironmentSampler."""
     occs, hops, targets = [], [], []
     L1, L2 = sampler.L1, sampler.L2
     count = 0
+
+    def record(a, b, before, psi_old, psi_new):
+        nonlocal count
+        ia, ib = a[0] * L2 + a[1], b[0] * L2 + b[1]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = np.clip(np.log(np.abs(psi_new) / np.abs(psi_old)), -12.0, 12.0)
+        occs.append(before.reshape(-1, L1, L2))
+        hops.append(bond_hop(before, ia, ib, L2, a[0] == b[0]))
+        targets.append(np.nan_to_num(t, nan=0.0))
+        count += len(before)
+
     for _ in range(sweeps):
-        for (a, b) in sampler.bonds:
-            idx, before, psi_old, psi_new = sampler._bond(a, b)
-            if len(idx) == 0:
-                continue
-            ia, ib = a[0] * L2 + a[1], b[0] * L2 + b[1]
-            with np.errstate(divide="ignore"):
-                t = np.clip(np.log(np.abs(psi_new) / np.abs(psi_old)), -12.0, 12.0)
-            occs.append(before.reshape(-1, L1, L2))
-            hops.append(bond_hop(before, ia, ib, L2, a[0] == b[0]))
-            targets.append(t)
-            count += len(idx)
+        if hasattr(sampler, "recorder"):
+            sampler.recorder = record
+            try:
+                sampler.sweep()
+            finally:
+                sampler.recorder = None
+        else:
+            for (a, b) in sampler.bonds:
+                idx, before, psi_old, psi_new = sampler._bond(a, b)
+                if len(idx):
+                    record(a, b, before, psi_old, psi_new)
         if log:
             log(f"training data: sweep done, {count} hops")
         if max_samples and count >= max_samples:

@@ -158,7 +158,8 @@ class TVSampler:
             psi_new = self.engine.evaluate(allp.reshape(-1, self.L1, self.L2))
             self.amplitudes += len(allp)
             own = np.concatenate(owners)
-            np.add.at(e, own, np.concatenate(coefs) * psi_new / psi[own])
+            with np.errstate(divide="ignore", invalid="ignore"):
+                np.add.at(e, own, np.concatenate(coefs) * psi_new / psi[own])
         return e
 
 
@@ -188,6 +189,8 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
     ``energy()`` (per-chain local energies) after each timed advance; returns
     ESS of the energy estimator, the sampling seconds (energy evaluation not
     included), the energy per site and tau."""
+    if samples < 1:
+        raise ValueError("measure_ess needs at least one sample")
     for i in range(burn_in):
         advance()
         if log:
@@ -204,11 +207,17 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
         if log:
             log(f"{len(energies)}/{samples} samples, {t_sample:.1f}s sampling")
     series = np.stack(energies, axis=1)
+    # a chain whose amplitude is exactly zero at some sample (every sector of
+    # one boundary-MPS cut truncated away) has no local energy there: it is
+    # left out of the estimator and counted
+    finite = np.all(np.isfinite(series), axis=1)
+    dropped = int(len(series) - finite.sum())
+    series = series[finite]
     tau = integrated_autocorrelation(series)
     ess = series.size / tau
     return {"ess": ess, "sample_s": t_sample, "ess_per_s": ess / max(t_sample, 1e-12),
             "energy_per_site": float(series.mean() / sites), "tau": tau, "chains": series.shape[0],
-            "samples": samples}
+            "chains_dropped": dropped, "samples": samples}
 
 
 def make_engine(lattice, **kw):
