@@ -53,7 +53,48 @@ def parse():
     ap.add_argument("--ess-epochs", type=int, default=6)
     ap.add_argument("--ess-learned-k", default="",
                     help="proposals per learned sample, comma-separated (default: bonds / 2)")
+    ap.add_argument("--sweep", default="", help="also time tetragono's sweep (cached-environment replace) and "
+                                                "its observer energy on this workload")
+    ap.add_argument("--sweep-only", action="store_true", help="time only the sweep and observer (one JSON line)")
+    ap.add_argument("--sweep-chains", type=int, default=256)
+    ap.add_argument("--sweep-count", type=int, default=2, help="timed sweeps after one untimed sweep")
     return ap.parse_args()
+
+
+def sweep_timing(args, log):
+    """Time EnvironmentSampler sweeps (tetragono's SweepSampling with
+    replace over cached environments) and one observer pass over all
+    chains, inputs resident, on ``args.sweep``."""
+    import numpy as np
+    import torch
+
+    from tnsp_amd import peps
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    spec = getattr(peps, args.sweep)
+    eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.lanes)
+    cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
+    start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
+    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
+    smp.sweep()
+    log("sweep: warm-up sweep done")
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for k in range(args.sweep_count):
+        smp.sweep()
+        log(f"sweep: {k + 1}/{args.sweep_count} timed sweeps")
+    torch.cuda.synchronize()
+    dt = (time.time() - t0) / args.sweep_count
+    t0 = time.time()
+    e = smp.local_energy()
+    torch.cuda.synchronize()
+    de = time.time() - t0
+    return {"workload": args.sweep, "chains": len(start), "lanes": args.lanes, "diagonal": args.ess_diagonal,
+            "sweep_s": dt, "chain_sweeps_per_s": len(start) / dt, "bonds": len(smp.bonds),
+            "proposals_per_s": smp.proposals / (args.sweep_count + 1) / dt,
+            "acceptance": smp.accepted / max(1, smp.proposals), "observer_s": de,
+            "energy_per_site": float(np.nanmean(e) / (spec.L1 * spec.L2))}
 
 
 def measured_traffic(kernel, args):
@@ -158,7 +199,7 @@ def ess_comparison(args, log):
     cand = peps.sample_configurations(spec, 4 * args.ess_chains, seed=77)
     psi = eng.evaluate(cand)
     start = cand[psi != 0][:args.ess_chains]
-    sweep = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal)
+    sweep = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
 
     def energy(configs):
         # tetragono's observer: psi(s') by replace over cached environments
@@ -216,6 +257,14 @@ def main():
     from tnsp_amd.engine import AmplitudeEngine
 
     torch.cuda.set_device(0)
+
+    def log(m):
+        print(f"[bench] {m}", file=sys.stderr, flush=True)
+
+    if args.sweep_only:
+        args.sweep = args.sweep or args.workload
+        print(json.dumps({"environment_sweep": sweep_timing(args, log)}), flush=True)
+        return
     spec = getattr(peps, args.workload)
     storages = peps.random_storages(spec)
     lat = peps.Lattice(spec, storages)
@@ -277,7 +326,8 @@ def main():
     nonzero = float(np.mean(psi != 0))
     ess = None
     if args.ess:
-        ess = ess_comparison(args, lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+        ess = ess_comparison(args, log)
+    env_sweep = sweep_timing(args, log) if args.sweep else None
     line = {
         "metric": "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec",
         "value": value,
@@ -306,7 +356,8 @@ def main():
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
                   "plain_path_configs_per_s": plain,
-                  "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned, "ess": ess},
+                  "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned, "ess": ess,
+                  "environment_sweep": env_sweep},
     }
     print(json.dumps(line), flush=True)
 
