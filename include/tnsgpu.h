@@ -91,16 +91,19 @@ int tns_svd_factor(tns_plan* p, int dtype, const void* input, void* workspace, i
 /* the same op for n tensors: planning spread over `threads` host threads,
  * execution as one launch per kernel kind for all n plans (the reference has
  * no batched form; each plan is the one tns_plan_contract / _qr / _svd
- * builds).  tns_plans_run: plans of one type; inputs n x (2 for contract, 1
- * for qr / svd), outputs n x (1 contract, 2 qr, 3 svd), workspaces n (each
- * tns_plan_workspace bytes).  Svd plans need tns_svd_factor_many first,
- * which applies one truncation rule to every tensor. */
+ * builds; tns_plan_conjugate_many: Tensor::conjugate, conjugate.hpp:32-120).
+ * tns_plans_run: plans of one type; inputs n x (2 for contract, 1 for qr /
+ * svd / conjugate), outputs n x (1 contract / conjugate, 2 qr, 3 svd),
+ * workspaces n (each tns_plan_workspace bytes).  Svd plans need
+ * tns_svd_factor_many first, which applies one truncation rule to every
+ * tensor. */
 int tns_plan_contract_many(int n, const tns_shape* const* a, const tns_shape* const* b, int npairs,
                            const int32_t* pairs, int threads, tns_plan** out);
 int tns_plan_qr_many(int n, const tns_shape* const* t, char direction, int nfree, const int32_t* free_names,
                      int32_t cq, int32_t cr, int threads, tns_plan** out);
 int tns_plan_svd_many(int n, const tns_shape* const* t, int nfree, const int32_t* free_u, int32_t cu, int32_t cv,
                       int32_t su, int32_t sv, int threads, tns_plan** out);
+int tns_plan_conjugate_many(int n, const tns_shape* const* t, int trivial_metric, int threads, tns_plan** out);
 int tns_plans_run(int n, tns_plan* const* plans, int dtype, const void* const* inputs, void* const* outputs,
                   void* const* workspaces, void* stream);
 int tns_svd_factor_many(int n, tns_plan* const* plans, int dtype, const void* const* inputs,

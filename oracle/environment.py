@@ -248,3 +248,51 @@ def local_energy(sites, config, cut, H):
 
         total += local_energy_term(sites, cfg, cut, pos, H, pool, wss_of=wss_of)
     return total
+
+
+def shrinker(site, occupation):
+    """lattice.py:291-318 ``_get_shrinker``: ``P`` the dimension-one edge of
+    the occupation, ``Q`` the conjugate of the physical edge, one unit entry."""
+    from . import tat
+
+    kind = site.kind
+    phys = site.edge_by_name("P0")
+    q = kind.make(occupation)
+    sh = tat.Tensor(kind, ["P", "Q"], [tat.Edge(kind, [(q, 1)], False), phys.conjugate()])
+    sh[{"Q": (kind.neg(q), 0), "P": (q, 0)}] = 1.0
+    return sh
+
+
+def holes(sites, config, cut):
+    """``Configuration.holes`` (sampling_lattice/lattice.py:362-416) with
+    ``hole(((l1, l2),))`` from the row environments (hint "H",
+    single_layer_auxiliaries.py:721-729): <psi|s|d_x psi> / <psi|s|psi> for
+    every site x, with the site's own edge names and its full physical
+    edge.  Returns a list of rows of tensors."""
+    cfg = np.array(config, dtype=np.int64)
+    L1, L2 = len(cfg), len(cfg[0])
+    env = Environments(project(sites, cfg), cut)
+    ws = env.hole()
+    w = value(ws)
+    inv_ws = (ws * (1.0 / (w * w))).conjugate()
+    all_name = {("T", "T")} | {(f"P_{a}_{b}_0", f"P_{a}_{b}_0") for a in range(L1) for b in range(L2)}
+    out = []
+    for l1 in range(L1):
+        up = env.up_to_down(l1 - 1)
+        down = env.down_to_up(l1 + 1)
+        row = []
+        for l2 in range(L2):
+            left = env.inline_left_tailed(l1, l2 - 1, up, down)
+            right = env.inline_right(l1, l2 + 1, up, down)
+            big = safe_contract(left, right, {("R1", "L1"), ("R3", "L3")})
+            hole = safe_rename(big, {"R2": "R0", "L2": "L0", "U": "U0", "D": "D0"})
+            names = set(all_name)
+            names.remove((f"P_{l1}_{l2}_0", f"P_{l1}_{l2}_0"))
+            if "T" not in hole.names:
+                names.remove(("T", "T"))
+            hole = hole.contract(inv_ws, names)
+            hole = safe_rename(hole, {"L0": "R", "R0": "L", "U0": "D", "D0": "U", f"P_{l1}_{l2}_0": "P0"})
+            hole = hole.contract(shrinker(sites[l1][l2], int(cfg[l1][l2])), {("P0", "P")}).edge_rename({"Q": "P0"})
+            row.append(hole)
+        out.append(row)
+    return out

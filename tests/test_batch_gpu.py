@@ -114,3 +114,21 @@ def test_batch_rename_take_select_values():
     full = batch.contract(ga, ca, [(n, n) for n in "ijkl"])
     want = [a.contract(a.conjugate(), {(n, n) for n in "ijkl"}).data[0] for a, _ in pairs]
     assert np.allclose(full.values(), want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_batch_conjugate_and_scale(kind):
+    from tnsp_amd import batch
+
+    pairs = items(kind, 5, seed=13)
+    ga = batch.TensorBatch.from_tensors([to_gpu(a) for a, _ in pairs])
+    for tm in (False, True):
+        cj = batch.conjugate(ga, trivial_metric=tm)
+        for i, (a, _) in enumerate(pairs):
+            o = a.conjugate(tm)
+            same_structure(cj.tensor(i), o)
+            assert np.array_equal(storage(cj.tensor(i)), o.data), (i, tm)
+    f = np.arange(1, len(pairs) + 1, dtype=np.float64) * 0.5
+    sc = batch.scale(ga, f)
+    for i, (a, _) in enumerate(pairs):
+        assert np.array_equal(storage(sc.tensor(i)), a.data * f[i])

@@ -135,3 +135,29 @@ def test_observer_energy_uncut_equals_scratch_energy():
         got = E.local_energy(sites, cfg, 10_000, H)
         want = sampling.local_energy(sites, cfg, 10_000, H)
         assert abs(got - want) <= 1e-10 * max(1.0, abs(want))
+
+
+def _pair(hole, site):
+    """Full contraction of a hole with its site tensor (one number)."""
+    t = hole.contract(site, {(n, n) for n in hole.names if n in site.names})
+    return float(t.data[0])
+
+
+@pytest.mark.parametrize("L1,L2,N,cut,rows", [(3, 3, 4, 10_000, None), (4, 4, 8, 3, [0])])
+def test_holes_contract_to_one(L1, L2, N, cut, rows):
+    """psi is linear in every site tensor, so <hole_x, x> = psi / psi = 1:
+    for every site without truncation; at a truncating cut for the sites of
+    row 0, whose environments are those of hole(()) itself."""
+    from oracle import environment as E
+
+    from oracle import peps
+
+    _, _, sites = small_lattice(L1, L2, N, seed=14)
+    cfg = next(c for c in configs(L1, L2, N, 8, seed=15) if peps.psi(sites, c, cut)[0] != 0)
+    hs = E.holes(sites, cfg, cut)
+    for l1 in (range(L1) if rows is None else rows):
+        for l2 in range(L2):
+            assert abs(_pair(hs[l1][l2], sites[l1][l2]) - 1.0) <= 1e-10, (l1, l2)
+    if rows is not None:
+        # the other rows use different truncations: not 1, which shows the cut bites
+        assert any(abs(_pair(hs[l1][0], sites[l1][0]) - 1.0) > 1e-6 for l1 in range(1, L1))

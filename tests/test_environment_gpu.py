@@ -97,3 +97,31 @@ def test_environment_observer_energy_uncut_equals_engine_energy():
     got = smp.local_energy()
     want = TVSampler(eng, start).local_energy()
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_environment_holes_match_oracle_truncated(lanes):
+    """Configuration.holes of every chain from the batched row environments
+    against the oracle, 4x4 at a truncating cut: same structure, values to
+    1e-10 relative."""
+    from oracle import environment as E, peps as opeps
+    from test_gpu_ops import rel, same_structure
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    L1, L2, N, cut = 4, 4, 8, 3
+    spec, storages, sites = small_lattice(L1, L2, N, seed=44)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    cand = configs(L1, L2, N, 12, seed=45)
+    start = np.stack([c for c in cand if opeps.psi(sites, c, cut)[0] != 0][:3])
+    smp = EnvironmentSampler(eng, start, cut=cut, lanes=lanes)
+    holes, psi = smp.holes()
+    for b, cfg in enumerate(start):
+        want = E.holes(sites, cfg, cut)
+        assert abs(psi[b] - E.value(E.Environments(opeps.project(sites, cfg), cut).hole())) <= 1e-10 * abs(psi[b])
+        for l1 in range(L1):
+            for l2 in range(L2):
+                g, o = holes[l1][l2].tensor(b), want[l1][l2]
+                same_structure(g, o)
+                assert rel(np.asarray(g.storage), o.data) < 1e-10, (b, l1, l2)

@@ -54,6 +54,7 @@ _SIGNATURES = {
                          ctypes.c_int, c_vpp],
     "tns_plan_svd_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_i32p, ctypes.c_int32, ctypes.c_int32,
                           ctypes.c_int32, ctypes.c_int32, ctypes.c_int, c_vpp],
+    "tns_plan_conjugate_many": [ctypes.c_int, c_vpp, ctypes.c_int, ctypes.c_int, c_vpp],
     "tns_plans_run": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, c_vpp, c_vp],
     "tns_svd_factor_many": [ctypes.c_int, c_vpp, ctypes.c_int, c_vpp, c_vpp, ctypes.c_int64, ctypes.c_double,
                             ctypes.c_int, c_vp],

@@ -99,6 +99,21 @@ class TensorBatch:
     def repeat(cls, tensor, n):
         return cls.select([tensor], np.zeros(n, dtype=np.int64))
 
+    @classmethod
+    def concat(cls, batches):
+        """The items of several batches of one name list, in order."""
+        b0 = batches[0]
+        bufs, idx, keep = [], [], []
+        for b in batches:
+            if b.names != b0.names:
+                raise ValueError("concat needs one name list")
+            idx.append(b.buf_idx + len(bufs))
+            bufs.extend(b.bufs)
+            keep.append(b.keep)
+        return cls(b0.model, b0.dtype, b0.names, np.concatenate([b.handles for b in batches]),
+                   np.concatenate([b.sizes for b in batches]), bufs, np.concatenate(idx),
+                   np.concatenate([b.offs for b in batches]), tuple(keep))
+
     def __len__(self):
         return len(self.handles)
 
@@ -241,6 +256,35 @@ def qr(t, direction, free_names, cq, cr):
     ws, wps = _workspaces(plans, t.dtype)
     _run(plans, t.dtype, t.ptrs, _interleave(q.ptrs, r.ptrs), wps)
     return q, r
+
+
+def conjugate(t, trivial_metric=False):
+    """Item-wise ``conjugate`` (TAT/include/TAT/implement/conjugate.hpp:32-120)."""
+    if t.model.kind.length == 0:
+        return t
+    n = len(t)
+    plans = _plans(n, _lib.lib().tns_plan_conjugate_many, _vpp(np.ascontiguousarray(t.handles)),
+                   int(bool(trivial_metric)))
+    (res,) = _outputs(plans, [0], t.model, t.dtype)
+    ws, wps = _workspaces(plans, t.dtype)
+    _run(plans, t.dtype, t.ptrs, res.ptrs, wps)
+    return res
+
+
+def scale(t, factors):
+    """Item i times ``factors[i]`` (a new batch; the scalar product of
+    PyTAT.hpp:707-718 per item)."""
+    torch = _torch()
+    n = len(t)
+    offs = np.zeros(n, dtype=np.int64)
+    np.cumsum(t.sizes[:-1], out=offs[1:])
+    total = int(t.sizes.sum())
+    buf = torch.empty(max(1, total), dtype=_tdtype(t.dtype), device=device())
+    if total:
+        f = np.repeat(np.asarray(factors, dtype=np.float64), t.sizes)
+        src = torch.cat([t.bufs[b][o:o + s] for b, o, s in zip(t.buf_idx, t.offs, t.sizes) if s > 0])
+        buf[:total] = src * torch.as_tensor(f, dtype=src.dtype, device=src.device)
+    return TensorBatch(t.model, t.dtype, t.names, t.handles, t.sizes, [buf], np.zeros(n), offs, t.keep)
 
 
 def svd(t, free_u, cu, cv, su, sv, cut=-1):

@@ -591,6 +591,23 @@ int tns_plan_svd_many(int n, const tns_shape* const* t, int nfree, const int32_t
     })
 }
 
+int tns_plan_conjugate_many(int n, const tns_shape* const* t, int trivial_metric, int threads, tns_plan** out) {
+    TNS_TRY({
+        for (int i = 0; i < n; i++) out[i] = nullptr;
+        try {
+            parallel_items(n, threads, [&](int i) {
+                auto p = std::make_unique<tns_plan>();
+                p->type = tns_plan::CONJ;
+                plan_conjugate(t[i]->s, trivial_metric != 0, p->eo_out, p->eo_tasks);
+                out[i] = p.release();
+            });
+        } catch (...) {
+            for (int i = 0; i < n; i++) delete out[i];
+            throw;
+        }
+    })
+}
+
 int tns_shapes_free(int n, tns_shape* const* shapes) {
     TNS_TRY({
         for (int i = 0; i < n; i++) delete shapes[i];
@@ -663,7 +680,21 @@ int tns_plans_run(int n, tns_plan* const* plans, int dtype, const void* const* i
         }
         auto ws = [&](int i) { return static_cast<char*>(workspaces[i]); };
         Batch bt(dtype);
-        if (type == tns_plan::CONTRACT) {
+        if (type == tns_plan::EDGEOP || type == tns_plan::CONJ) {
+            // permute-only plans: zero what no block copy covers, then copy
+            for (int i = 0; i < n; i++) {
+                void* b[2] = {const_cast<void*>(inputs[i]), outputs[i]};
+                int64_t covered = 0;
+                for (auto& t : plans[i]->eo_tasks) covered += t.n;
+                if (covered < plans[i]->eo_out.size) bt.fill({FillTask{1, 0, plans[i]->eo_out.size}}, b, es);
+            }
+            bt.flush_fill();
+            for (int i = 0; i < n; i++) {
+                void* b[2] = {const_cast<void*>(inputs[i]), outputs[i]};
+                bt.perm(plans[i]->eo_tasks, b, es);
+            }
+            bt.flush_perm();
+        } else if (type == tns_plan::CONTRACT) {
             auto bufs = [&](int i, void** b) {
                 b[0] = const_cast<void*>(inputs[2 * i]);
                 b[1] = const_cast<void*>(inputs[2 * i + 1]);
@@ -751,7 +782,7 @@ int tns_plans_run(int n, tns_plan* const* plans, int dtype, const void* const* i
             }
             bt.flush_diag();
         } else {
-            throw Error("tns_plans_run handles contract, qr and svd plans");
+            throw Error("tns_plans_run: unknown plan type");
         }
         bt.run(as_stream(stream));
     })

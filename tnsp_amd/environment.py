@@ -66,6 +66,9 @@ class EnvironmentSampler:
         self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
                        for l2 in range(self.L2)] for l1 in range(self.L1)]
         self._one = _one(spec, self.lattice.dtype)
+        # made here, on the caller's stream, before any lane reads them
+        self._shr = [[(self.lattice.shrinker(l1, l2, 0), self.lattice.shrinker(l1, l2, 1))
+                      for l2 in range(self.L2)] for l1 in range(self.L1)]
         self.psi = np.abs(engine.evaluate(self.configs, cut=self.cut))
         if np.any(self.psi == 0):
             raise ValueError("initial configurations with psi = 0")
@@ -339,6 +342,83 @@ class EnvironmentSampler:
 
         self._in_lanes(len(self.configs), run)
         return self.configs
+
+    # -- holes (sampling_lattice/lattice.py:362-416) ------------------------------------
+    def _holes_group(self):
+        """Holes of this object's chains: every site's hole(((l1, l2),))
+        from the row environments (single_layer_auxiliaries.py:721-729),
+        then the normalisation, renaming and physical expansion of
+        ``Configuration.holes``."""
+        L1, L2 = self.L1, self.L2
+        B = len(self.configs)
+        one = self._ones(B)
+        down = {L1: [one] * L2}
+        for l1 in reversed(range(1, L1)):
+            down[l1] = self._two_line("DULR", down[l1 + 1], self._row(l1))
+        up = [one] * L2
+        raw = [[None] * L2 for _ in range(L1)]
+        ws = None
+        for l1 in range(L1):
+            dn = down[l1 + 1]
+            row = self._row(l1)
+            rt, tailed = {}, None
+            for c in reversed(range(L2 + 1)):
+                if c == L2:
+                    right = one
+                else:
+                    right = Bt.contract(tailed, row[c].rename({"L": "L2"}), [("L2", "R"), ("U", "D")])
+                    right = Bt.contract(right, up[c].rename({"L": "L1"}), [("L1", "R"), ("U", "D")])
+                tailed = right if c == 0 else Bt.contract(right, dn[c - 1], [("L3", "R")]).rename({"L": "L3"})
+                rt[c] = tailed
+            if l1 == 0:
+                ws = rt[0]
+            lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
+            for l2 in range(L2):
+                big = Bt.contract(lt, rt[l2 + 1], [("R1", "L1"), ("R3", "L3")])
+                raw[l1][l2] = big.rename({"R2": "R0", "L2": "L0", "U": "U0", "D": "D0"})
+                if l2 < L2 - 1:
+                    left = Bt.contract(lt, row[l2].rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
+                    left = Bt.contract(left, dn[l2].rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
+                    lt = Bt.contract(left, up[l2 + 1], [("R1", "L")]).rename({"R": "R1"})
+            if l1 < L1 - 1:
+                up = self._two_line("UDLR", up, row)
+        w = ws.values()
+        with np.errstate(divide="ignore"):
+            factor = np.where(w != 0, 1.0 / (w * w), 0.0)
+        inv_ws = Bt.conjugate(Bt.scale(ws, factor))
+        physics = [f"P_{a}_{b}_0" for a in range(L1) for b in range(L2)]
+        out = [[None] * L2 for _ in range(L1)]
+        for l1 in range(L1):
+            for l2 in range(L2):
+                h = raw[l1][l2]
+                own = f"P_{l1}_{l2}_0"
+                pairs = [(n, n) for n in physics if n != own]
+                if "T" in h.names:
+                    pairs.append(("T", "T"))
+                h = Bt.contract(h, inv_ws, pairs)
+                h = h.rename({"L0": "R", "R0": "L", "U0": "D", "D0": "U", own: "P0"})
+                sh = Bt.TensorBatch.select(self._shr[l1][l2], self.configs[:, l1, l2])
+                out[l1][l2] = Bt.contract(h, sh, [("P0", "P")]).rename({"Q": "P0"})
+        return out, w
+
+    def holes(self, configs=None):
+        """``Configuration.holes`` (lattice.py:362-416) of every chain:
+        <psi|s|d_x psi> / <psi|s|psi> for each site x, a tensor with the
+        site's own edge names and full physical edge.  Returns (holes,
+        psi): ``holes[l1][l2]`` a TensorBatch over the chains, ``psi`` the
+        hole(()) values (a chain with psi = 0 gets zero holes)."""
+        cfg = self.configs if configs is None else np.ascontiguousarray(np.asarray(configs), dtype=np.int8)
+        parts = {}
+
+        def run(child, idx):
+            child.configs = cfg[idx].copy()
+            parts[int(idx[0])] = child._holes_group()
+
+        self._in_lanes(len(cfg), run)
+        order = [parts[k] for k in sorted(parts)]
+        holes = [[Bt.TensorBatch.concat([p[0][l1][l2] for p in order]) for l2 in range(self.L2)]
+                 for l1 in range(self.L1)]
+        return holes, np.concatenate([p[1] for p in order])
 
     # -- lanes --------------------------------------------------------------------------
     def _child(self):

@@ -164,6 +164,18 @@ class Lattice:
         key = (l1, l2, int(occupation))
         t = self._projected.get(key)
         if t is None:
+            sh = self.shrinker(l1, l2, occupation)
+            t = self.sites[l1][l2].contract(sh.edge_rename({"P": f"P_{l1}_{l2}_0"}), {("P0", "Q")})
+            self._projected[key] = t
+        return t
+
+    def shrinker(self, l1, l2, occupation):
+        """``_get_shrinker`` (lattice.py:291-318): ``P`` the dimension-one
+        edge of the occupation, ``Q`` the conjugate physical edge, one unit
+        entry."""
+        key = ("shrinker", l1, l2, int(occupation))
+        sh = self._projected.get(key)
+        if sh is None:
             site = self.sites[l1][l2]
             kind = KINDS[self.spec.kind]
             mod = getattr(TAT, self.spec.kind)
@@ -172,9 +184,8 @@ class Lattice:
             q = kind.make(occupation)
             sh = smod.Tensor(["P", "Q"], [Edge(kind, [(q, 1)], False), phys.conjugate()]).zero_()
             sh[{"Q": (kind.neg(q), 0), "P": (q, 0)}] = 1.0
-            t = site.contract(sh.edge_rename({"P": f"P_{l1}_{l2}_0"}), {("P0", "Q")})
-            self._projected[key] = t
-        return t
+            self._projected[key] = sh
+        return sh
 
 
 def _safe_contract(t1, t2, pairs):
