@@ -25,6 +25,8 @@
 //   reverse <out> <a> <n> names... <apply_parity 0|1>
 //   conjugate <out> <a>
 //   print <id>
+//   dump <id> <path>            Tensor::dump() bytes to a file
+//   load <id> <path>            Tensor::load() of a file's bytes
 //   amplitude <out> <L1> <L2> <cut> <prefix>   sites <prefix>_<l1>_<l2> already
 //       shrunk (physical legs contracted away); computes psi with tetragono's
 //       contraction order and stores it as tensor <out>.
@@ -36,6 +38,8 @@
 //       psi; projection and amplitude are both inside the timed region
 #include <chrono>
 #include <cstdio>
+#include <fstream>
+#include <iterator>
 #include <iostream>
 #include <map>
 #include <sstream>
@@ -318,6 +322,20 @@ struct Interp {
                 std::string out, a;
                 in >> out >> a;
                 env.insert_or_assign(out, env.at(a).conjugate());
+            } else if (cmd == "dump") {
+                std::string id, path;
+                in >> id >> path;
+                const std::string bytes = env.at(id).dump();
+                std::ofstream f(path, std::ios::binary);
+                f.write(bytes.data(), std::streamsize(bytes.size()));
+            } else if (cmd == "load") {
+                std::string id, path;
+                in >> id >> path;
+                std::ifstream f(path, std::ios::binary);
+                const std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+                T t;
+                t.load(bytes);
+                env.insert_or_assign(id, std::move(t));
             } else if (cmd == "print") {
                 std::string id;
                 in >> id;

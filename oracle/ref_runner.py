@@ -66,7 +66,7 @@ def parse(kind, text):
             for r in range(rank):
                 el = toks[i + 1 + r].split()
                 name, nseg, arrow = el[1], int(el[2]), bool(int(el[3]))
-                segs = [(int(el[4 + 2 * s]), int(el[5 + 2 * s])) for s in range(nseg)]
+                segs = [(int(el[4 + 2 * s]) if kind.length else None, int(el[5 + 2 * s])) for s in range(nseg)]
                 names.append(name)
                 edges.append(tat.Edge(kind, segs, arrow))
             dl = toks[i + 1 + rank].split()

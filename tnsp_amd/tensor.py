@@ -438,6 +438,36 @@ class Tensor:
         plan.run(self.dtype, [self._data.data_ptr()], [o._data.data_ptr() for o in outs], wp, _stream())
         return tuple(outs)
 
+    # ------------------------------------------------------------ binary I/O
+    def dump(self):
+        """PyTAT ``dump`` (PyTAT/PyTAT.hpp:757-760): TAT's binary format
+        (TAT/include/TAT/miscellaneous/io.hpp:687-705) as bytes."""
+        from . import tatio
+
+        return tatio.serialize(self.model.kind, self.names, self.edges, np.asarray(self.storage))
+
+    def load(self, data):
+        """PyTAT ``load`` (PyTAT/PyTAT.hpp:761-765): this tensor becomes the
+        one stored in ``data`` (binary version 1, or version 0 whose blocks
+        are reordered as io.hpp:742-749 does); returns self."""
+        from . import tatio
+
+        names, edges, storage, version = tatio.parse(self.model.kind, data, self.dtype)
+        t = Tensor(self.model, self.dtype, names, edges)
+        if storage.size != t._shape.size:
+            raise RuntimeError("TAT tensor bytes: storage size does not match the edges")
+        if version == 0:
+            blocks = [(tuple(edges[i].segments[p][0] for i, p in enumerate(pos)), int(np.prod(shape)))
+                      for pos, _, shape in t._shape.blocks()]
+            storage = tatio.v0_to_v1(blocks, storage)
+        t.storage = storage
+        self._shape, self._data = t._shape, t._data
+        return self
+
+    def __reduce__(self):
+        # PyTAT pickles through dump / load (PyTAT/PyTAT.hpp:768-771)
+        return (_from_bytes, (self.model.kind.name, self.dtype, self.dump()))
+
     def conjugate(self, trivial_metric=False):
         if self.model.kind.length == 0:
             return self.copy()
@@ -448,3 +478,10 @@ class Tensor:
         return f"{self.model.name}.{self.dtype}.Tensor(names={self.names}, edges={self.edges})"
 
     __str__ = __repr__
+
+
+def _from_bytes(kind_name, dtype, data):
+    from . import TAT
+
+    sym = getattr(TAT, kind_name)
+    return (sym.D if dtype == "float64" else sym.S).Tensor([], []).load(data)
