@@ -77,12 +77,12 @@ def sweep_timing(args, log):
     cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
     smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
-    smp.sweep()
+    smp.sweep(log=lambda m: log(f"sweep (warm-up): {m}"))
     log("sweep: warm-up sweep done")
     torch.cuda.synchronize()
     t0 = time.time()
     for k in range(args.sweep_count):
-        smp.sweep()
+        smp.sweep(log=lambda m: log(f"sweep {k + 1}: {m}"))
         log(f"sweep: {k + 1}/{args.sweep_count} timed sweeps")
     torch.cuda.synchronize()
     dt = (time.time() - t0) / args.sweep_count
@@ -93,7 +93,7 @@ def sweep_timing(args, log):
     return {"workload": args.sweep, "chains": len(start), "lanes": args.lanes, "diagonal": args.ess_diagonal,
             "sweep_s": dt, "chain_sweeps_per_s": len(start) / dt, "bonds": len(smp.bonds),
             "proposals_per_s": smp.proposals / (args.sweep_count + 1) / dt,
-            "acceptance": smp.accepted / max(1, smp.proposals), "observer_s": de,
+            "acceptance": smp.accepted / max(1, smp.proposals), "hole_refreshes": smp.refreshed, "observer_s": de,
             "energy_per_site": float(np.nanmean(e) / (spec.L1 * spec.L2))}
 
 

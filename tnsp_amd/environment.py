@@ -146,10 +146,16 @@ class EnvironmentSampler:
             self.configs[take, a[0], a[1]], self.configs[take, b[0], b[1]] = nb[take], na[take]
             self.psi[take] = wss[acc]
             self.accepted += int(acc.sum())
+            self._hole_cache[take] = np.nan
         if len(dbl) and self.diagonal == "hole":
-            with self._lock:  # one engine handle: one caller at a time
-                wss = np.abs(self.engine.evaluate(self.configs[dbl], cut=self.cut))
-            self.refreshed += len(dbl)
+            # hole(()) is recomputed only for chains that moved since it was
+            # last known, as the lazy graph's cache does
+            need = dbl[np.isnan(self._hole_cache[dbl])]
+            if len(need):
+                with self._lock:  # one engine handle: one caller at a time
+                    self._hole_cache[need] = np.abs(self.engine.evaluate(self.configs[need], cut=self.cut))
+                self.refreshed += len(need)
+            wss = self._hole_cache[dbl]
             with np.errstate(divide="ignore", invalid="ignore"):
                 acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
             self.psi[dbl[acc]] = wss[acc]
@@ -182,6 +188,7 @@ class EnvironmentSampler:
                     self._hole = rt[0].values()
                 else:
                     self.psi = np.abs(rt[0].values())
+                    self._hole_cache = self.psi.copy()
             # _inline_left_to_right_tailed[l1, -1] (:254-282)
             lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
             for l2 in range(L2 - 1):
@@ -334,6 +341,7 @@ class EnvironmentSampler:
         def run(child, idx):
             child.configs = self.configs[idx].copy()
             child.psi = self.psi[idx].copy()
+            child._hole_cache = np.full(len(idx), np.nan)
             child._u = self._u[:, idx]
             child._horizontal(log)
             child._vertical(log)
