@@ -205,6 +205,14 @@ def ess_comparison(args, log):
         # tetragono's observer: psi(s') by replace over cached environments
         return sweep.local_energy(configs)
 
+    bonds_a = np.array([a[0] * spec.L2 + a[1] for a, _ in sweep.bonds])
+    bonds_b = np.array([b[0] * spec.L2 + b[1] for _, b in sweep.bonds])
+
+    def density(configs):
+        # the interaction energy V sum n_i n_j: bounded, a second observable
+        flat = np.asarray(configs).reshape(len(configs), -1).astype(np.int64)
+        return (flat[:, bonds_a] * flat[:, bonds_b]).sum(1).astype(np.float64)
+
     t0 = time.time()
     occs, hops, targets = collect_training_data(sweep, args.ess_train_sweeps, log=log)
     t_collect = time.time() - t0
@@ -214,14 +222,14 @@ def ess_comparison(args, log):
     t_train = time.time() - t0
     res = {}
     res_sweep = measure_ess(sweep.sweep, lambda: energy(sweep.configs), args.ess_samples, sites=L,
-                            log=lambda m: log(f"environment sweep: {m}"))
+                            log=lambda m: log(f"environment sweep: {m}"), extra=lambda: density(sweep.configs))
     res_sweep["acceptance"] = sweep.accepted / max(1, sweep.proposals)
     res_sweep["diagonal"] = args.ess_diagonal
     res["sweep"] = res_sweep
     if args.ess_engine_sweep:
         plain = TVSampler(eng, sweep.configs.copy(), seed=1)
         r = measure_ess(plain.sweep, lambda: energy(plain.configs), args.ess_samples, sites=L,
-                        log=lambda m: log(f"engine sweep: {m}"))
+                        log=lambda m: log(f"engine sweep: {m}"), extra=lambda: density(plain.configs))
         r["acceptance"] = plain.accepted / max(1, plain.proposals)
         res["engine_sweep"] = r
     ks = [int(x) for x in args.ess_learned_k.split(",") if x] or [max(1, len(sweep.bonds) // 2)]
@@ -230,13 +238,15 @@ def ess_comparison(args, log):
         learned = LearnedSampler(eng, net, sweep.configs.copy(), seed=2)
         r = measure_ess(lambda: [learned.step() for _ in range(K)],
                         lambda: energy(learned.configs), args.ess_samples, sites=L,
-                        log=lambda m: log(f"learned K={K}: {m}"))
+                        log=lambda m: log(f"learned K={K}: {m}"), extra=lambda: density(learned.configs))
         r["acceptance"] = learned.accepted / max(1, learned.proposals)
         r["proposals_per_sample"] = K
         res_learned.append(r)
     best = max(res_learned, key=lambda r: r["ess_per_s"])
+    best_x = max(res_learned, key=lambda r: r["ess_extra_per_s"])
     out = {"workload": args.ess, "chains": len(start), "learned": res_learned,
            "gain": best["ess_per_s"] / max(res_sweep["ess_per_s"], 1e-12),
+           "gain_density": best_x["ess_extra_per_s"] / max(res_sweep["ess_extra_per_s"], 1e-12),
            "training": {"samples": int(len(hops)), "collect_s": t_collect, "train_s": t_train,
                         "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None}}
     out.update(res)

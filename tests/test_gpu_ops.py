@@ -179,6 +179,30 @@ def test_qr_large_blocks(kind, direction):
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_svd_register_sectors(dtype):
+    """Sectors of 100-190 rows take the register-resident Jacobi kernel
+    (per-op SVDs route like the engine's): singular values and U S V match
+    the oracle, 1e-10 in float64 and 1e-4 in float32, truncated and not."""
+    from oracle import tat
+
+    K = tat.KINDS["BoseU1"]
+    rng = np.random.default_rng(8)
+    a = tat.Tensor(K, ["a", "b"], [([(0, 150), (1, 120), (-1, 190)], False), ([(0, 130), (-1, 100), (1, 170)], False)])
+    a.data[:] = rng.standard_normal(a.size)
+    g = to_gpu(a, dtype)
+    tol = 1e-10 if dtype == "float64" else 1e-4
+    for cut in (-1, 150):
+        u, s, v = a.svd({"a"}, "cu", "cv", "su", "sv", cut)
+        gu, gs, gv = g.svd({"a"}, "cu", "cv", "su", "sv", cut)
+        for x, o in ((gu, u), (gs, s), (gv, v)):
+            same_structure(x, o)
+        assert rel(storage(gs), s.data) < tol
+        r0 = u.contract(s, {("cu", "su")}).contract(v, {("sv", "cv")})
+        r1 = gu.contract(gs, {("cu", "su")}).contract(gv, {("sv", "cv")})
+        assert rel(storage(r1), r0.data) < tol
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
 def test_contract_every_gemm_tile(dtype):
     """Sectors of 3 .. 130 rows / columns exercise the 16x16, 32x32 and 64x64
     GEMM tiles (size classes <= 16, <= 48, larger)."""

@@ -184,18 +184,20 @@ def integrated_autocorrelation(series, c=5.0):
     return max(tau, 1e-9)
 
 
-def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
+def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None, extra=None):
     """Calls ``advance()`` ``burn_in`` + ``samples`` times, recording
     ``energy()`` (per-chain local energies) after each timed advance; returns
     ESS of the energy estimator, the sampling seconds (energy evaluation not
-    included), the energy per site and tau."""
+    included), the energy per site and tau.  ``extra()``, if given, is a
+    second per-chain observable recorded alongside (its ESS is reported as
+    ``ess_extra``)."""
     if samples < 1:
         raise ValueError("measure_ess needs at least one sample")
     for i in range(burn_in):
         advance()
         if log:
             log(f"burn-in {i + 1}/{burn_in}")
-    energies = []
+    energies, extras = [], []
     t_sample = 0.0
     for _ in range(samples):
         t0 = time.time()
@@ -204,6 +206,8 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
         if log:
             log(f"sample {len(energies) + 1}/{samples} drawn, {t_sample:.1f}s sampling")
         energies.append(energy())
+        if extra is not None:
+            extras.append(extra())
         if log:
             log(f"{len(energies)}/{samples} samples, {t_sample:.1f}s sampling")
     series = np.stack(energies, axis=1)
@@ -215,9 +219,16 @@ def measure_ess(advance, energy, samples, burn_in=2, sites=1, log=None):
     series = series[finite]
     tau = integrated_autocorrelation(series)
     ess = series.size / tau
-    return {"ess": ess, "sample_s": t_sample, "ess_per_s": ess / max(t_sample, 1e-12),
-            "energy_per_site": float(series.mean() / sites), "tau": tau, "chains": series.shape[0],
-            "chains_dropped": dropped, "samples": samples}
+    out = {"ess": ess, "sample_s": t_sample, "ess_per_s": ess / max(t_sample, 1e-12),
+           "energy_per_site": float(series.mean() / sites), "energy_median_per_site": float(np.median(series) / sites),
+           "tau": tau, "chains": series.shape[0], "chains_dropped": dropped, "samples": samples}
+    if extras:
+        xs = np.stack(extras, axis=1)
+        tau_x = integrated_autocorrelation(xs)
+        out.update({"tau_extra": tau_x, "ess_extra": xs.size / tau_x,
+                    "ess_extra_per_s": xs.size / tau_x / max(t_sample, 1e-12),
+                    "extra_per_site": float(xs.mean() / sites)})
+    return out
 
 
 def make_engine(lattice, **kw):
