@@ -234,8 +234,14 @@ def ess_comparison(args, log):
         res["engine_sweep"] = r
     ks = [int(x) for x in args.ess_learned_k.split(",") if x] or [max(1, len(sweep.bonds) // 2)]
     res_learned = []
+    # the learned chains continue from where the sweep left them; a chain the
+    # sweep left at hole(()) = 0 (every sector truncated away) restarts from
+    # its initial configuration
+    lstart = sweep.configs.copy()
+    zero = eng.evaluate(lstart) == 0
+    lstart[zero] = start[zero]
     for K in ks:
-        learned = LearnedSampler(eng, net, sweep.configs.copy(), seed=2)
+        learned = LearnedSampler(eng, net, lstart.copy(), seed=2)
         r = measure_ess(lambda: [learned.step() for _ in range(K)],
                         lambda: energy(learned.configs), args.ess_samples, sites=L,
                         log=lambda m: log(f"learned K={K}: {m}"), extra=lambda: density(learned.configs))

@@ -44,7 +44,14 @@ from .sampler import bond_order
 
 class EnvironmentSampler:
     """Lockstep sweep sampling of B chains of a FermiU1 or FermiZ2 PEPS with
-    the tV Hamiltonian (hops t, density interaction V)."""
+    the tV Hamiltonian (hops t, density interaction V).
+
+    ``sweep()`` is ``SweepSampling.__call__``; ``local_energy()`` the
+    observer's energy and ``holes()`` ``Configuration.holes``, all over the
+    same cached environments.  ``cut`` overrides the PEPS's boundary-MPS
+    bond dimension; ``diagonal`` chooses how doubly occupied bonds are
+    treated (see the module notes); ``lanes`` splits the chains into groups
+    that run on their own host threads and HIP streams."""
 
     def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole", lanes=1):
         if diagonal not in ("hole", "keep"):
