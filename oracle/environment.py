@@ -296,3 +296,32 @@ def holes(sites, config, cut):
             row.append(hole)
         out.append(row)
     return out
+
+
+def energy_gradient(sites, configs, cut, H, reweight=None):
+    """observer.py:396-407, 543-555: 2 conj(<E Delta> - E <Delta>) per site
+    (conj with the trivial metric, utility.py:230-231)
+    with Delta = ``holes``, E the observer energy through ``replace``,
+    averages weighted by ``reweight``; a configuration with psi = 0 is
+    skipped.  Returns (gradient rows, energy)."""
+    es, hs, ws = [], [], []
+    for i, cfg in enumerate(configs):
+        e = local_energy(sites, cfg, cut, H)
+        if e is None:
+            continue
+        es.append(e)
+        hs.append(holes(sites, cfg, cut))
+        ws.append(1.0 if reweight is None else float(reweight[i]))
+    ws = np.array(ws) / np.sum(ws)
+    energy = float(np.sum(ws * np.array(es)))
+    out = []
+    for l1 in range(len(hs[0])):
+        row = []
+        for l2 in range(len(hs[0][0])):
+            g = hs[0][l1][l2] * 0.0
+            for w, e, h in zip(ws, es, hs):
+                g = g + h[l1][l2] * (2.0 * w * (e - energy))
+            # lattice_conjugate (utility.py:230-231): conjugate with the trivial metric
+            row.append(g.conjugate(True))
+        out.append(row)
+    return out, energy

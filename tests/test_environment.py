@@ -161,3 +161,51 @@ def test_holes_contract_to_one(L1, L2, N, cut, rows):
     if rows is not None:
         # the other rows use different truncations: not 1, which shows the cut bites
         assert any(abs(_pair(hs[l1][0], sites[l1][0]) - 1.0) > 1e-6 for l1 in range(1, L1))
+
+
+def test_energy_gradient_matches_finite_differences():
+    """The observer's gradient 2 conj(<E Delta> - E <Delta>) with exact
+    enumeration weights |psi|^2 is the derivative of the variational energy
+    E(T) = sum_s psi^2 E_loc / sum_s psi^2 with respect to each site tensor's
+    storage (after bringing the gradient to the site's edge order); checked
+    by central differences on a 2x3 lattice at half filling, no truncation."""
+    import itertools
+
+    from oracle import environment as E, peps, sampling
+
+    from test_sampler import OracleAmplitudes
+    from tnsp_amd.sampler import TVSampler
+
+    L1, L2, N, cut = 2, 3, 3, 10_000
+    spec, _, sites = small_lattice(L1, L2, N, seed=16)
+    H = sampling.tv_hamiltonian(1.0, 1.0)
+    cfgs = []
+    for occ in itertools.combinations(range(L1 * L2), N):
+        c = np.zeros(L1 * L2, dtype=np.int64)
+        c[list(occ)] = 1
+        cfgs.append(c.reshape(L1, L2))
+    cfgs = [c for c in cfgs if peps.psi(sites, c, cut)[0] != 0]
+
+    def energy(st):
+        # the closed-form local energy (pinned against the observer contraction
+        # in test_sampler) over every configuration, weights psi^2
+        amps = OracleAmplitudes(spec, st)
+        smp = TVSampler(amps, np.stack(cfgs))
+        ps = smp.psi
+        return float(np.sum(ps ** 2 * smp.local_energy()) / np.sum(ps ** 2)), ps
+
+    e0, ps = energy(sites)
+    grad, eg = E.energy_gradient(sites, cfgs, cut, H, reweight=ps ** 2)
+    assert abs(eg - e0) <= 1e-10 * abs(e0)
+    rng = np.random.default_rng(17)
+    eps = 1e-5
+    for l1, l2 in ((0, 0), (1, 1), (0, 2)):
+        g = grad[l1][l2].transpose(sites[l1][l2].names)
+        assert [e.segments for e in g.edges] == [e.segments for e in sites[l1][l2].edges]
+        for i in rng.choice(sites[l1][l2].size, 2, replace=False):
+            plus = [[t.copy() for t in row] for row in sites]
+            minus = [[t.copy() for t in row] for row in sites]
+            plus[l1][l2].data[i] += eps
+            minus[l1][l2].data[i] -= eps
+            fd = (energy(plus)[0] - energy(minus)[0]) / (2 * eps)
+            assert abs(g.data[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (l1, l2, i, g.data[i], fd)

@@ -125,3 +125,29 @@ def test_environment_holes_match_oracle_truncated(lanes):
                 g, o = holes[l1][l2].tensor(b), want[l1][l2]
                 same_structure(g, o)
                 assert rel(np.asarray(g.storage), o.data) < 1e-10, (b, l1, l2)
+
+
+def test_environment_energy_gradient_matches_oracle():
+    """The observer's energy gradient over chains (holes, replace energies,
+    trivial-metric conjugate) against the oracle, 4x4 at a truncating cut,
+    with non-uniform weights."""
+    from oracle import environment as E, peps as opeps, sampling
+    from test_gpu_ops import rel, same_structure
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    L1, L2, N, cut = 4, 4, 8, 3
+    spec, storages, sites = small_lattice(L1, L2, N, seed=46)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    cand = configs(L1, L2, N, 12, seed=47)
+    start = np.stack([c for c in cand if opeps.psi(sites, c, cut)[0] != 0][:3])
+    w = np.array([1.0, 0.5, 2.0])
+    smp = EnvironmentSampler(eng, start, cut=cut, lanes=2)
+    grad, energy = smp.energy_gradient(reweight=w)
+    want, e_want = E.energy_gradient(sites, start, cut, sampling.tv_hamiltonian(1.0, 1.0), reweight=w)
+    assert abs(energy - e_want) <= 1e-9 * max(1.0, abs(e_want))
+    for l1 in range(L1):
+        for l2 in range(L2):
+            same_structure(grad[l1][l2], want[l1][l2])
+            assert rel(np.asarray(grad[l1][l2].storage), want[l1][l2].data) < 1e-8, (l1, l2)

@@ -435,6 +435,42 @@ class EnvironmentSampler:
                  for l1 in range(self.L1)]
         return holes, np.concatenate([p[1] for p in order])
 
+    def energy_gradient(self, configs=None, reweight=None):
+        """The observer's energy gradient (observer.py:396-407, 543-555) over
+        the chains: 2 conj(<E Delta> - E <Delta>) per site (conj with the
+        trivial metric, utility.py:230-231), Delta the holes,
+        averages weighted by ``reweight`` (default 1: the chains sample
+        |psi|^2); chains with psi(s) = 0 are skipped as the observer skips
+        them.  Returns (gradient, energy): ``gradient[l1][l2]`` a drop-in
+        Tensor with the site's edges, ``energy`` the weighted mean local
+        energy."""
+        torch = Bt._torch()
+        holes, psi = self.holes(configs)
+        e = self.local_energy(configs)
+        w = np.ones(len(e)) if reweight is None else np.asarray(reweight, dtype=np.float64).copy()
+        w[(psi == 0) | ~np.isfinite(e)] = 0.0
+        if w.sum() <= 0:
+            raise ValueError("no chain with a non-zero amplitude")
+        w = w / w.sum()
+        energy = float(np.sum(w * np.where(w > 0, e, 0.0)))
+        coef = np.where(w > 0, 2.0 * w * (e - energy), 0.0)
+        out = []
+        for l1 in range(self.L1):
+            row = []
+            for l2 in range(self.L2):
+                h = holes[l1][l2]
+                if len(set(h.sizes.tolist())) != 1:
+                    raise RuntimeError("holes of one site differ in structure between chains")
+                n = int(h.sizes[0])
+                stack = torch.stack([h.bufs[b][o:o + n] for b, o in zip(h.buf_idx, h.offs)])
+                c = torch.as_tensor(coef, dtype=stack.dtype, device=stack.device)
+                g = h.tensor(0)
+                g._data = (c[:, None] * stack).sum(0)
+                # lattice_conjugate (utility.py:230-231): the trivial metric
+                row.append(g.conjugate(trivial_metric=True))
+            out.append(row)
+        return out, energy
+
     # -- lanes --------------------------------------------------------------------------
     def _child(self):
         """A view of this sampler for one group of chains: shares the engine,
