@@ -325,3 +325,53 @@ def energy_gradient(sites, configs, cut, H, reweight=None):
             row.append(g.conjugate(True))
         out.append(row)
     return out, energy
+
+
+def natural_gradient(sites, configs, cut, H, step, error=0.0, reweight=None):
+    """observer.py:576-675 (real tensors): conjugate gradient on
+    (D^T D) x = D^T E with D = sqrt(w_s) (Delta_s - <Delta>), E = sqrt(w_s)
+    (E_s - <E>); returns (2 x as site tensors through the trivial-metric
+    conjugate, energy, steps, the D matrix, the E vector and 2 x itself)."""
+    es, hs, ws = [], [], []
+    for i, cfg in enumerate(configs):
+        e = local_energy(sites, cfg, cut, H)
+        if e is None:
+            continue
+        es.append(e)
+        hs.append(holes(sites, cfg, cut))
+        ws.append(1.0 if reweight is None else float(reweight[i]))
+    ws = np.array(ws) / np.sum(ws)
+    es = np.array(es)
+    energy = float(np.sum(ws * es))
+    L1, L2 = len(hs[0]), len(hs[0][0])
+    arr = np.array([np.concatenate([h[l1][l2].data for l1 in range(L1) for l2 in range(L2)]) for h in hs])
+    mean = (ws[:, None] * arr).sum(0)
+    D = (arr - mean[None, :]) * np.sqrt(ws)[:, None]
+    E = np.sqrt(ws) * (es - energy)
+    b = D.T @ E
+    b2 = float(b @ b)
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    r2 = float(r @ r)
+    t = 0
+    while t < step and not (error != 0.0 and error ** 2 > r2 / b2):
+        Dp = D @ p
+        alpha = r2 / float(Dp @ Dp)
+        x = x + alpha * p
+        r = r - alpha * (D.T @ Dp)
+        nr2 = float(r @ r)
+        p = r + (nr2 / r2) * p
+        r2 = nr2
+        t += 1
+    x = 2 * x
+    out, off = [], 0
+    for l1 in range(L1):
+        row = []
+        for l2 in range(L2):
+            g = hs[0][l1][l2].copy()
+            g.data[:] = x[off:off + g.size]
+            off += g.size
+            row.append(g.conjugate(True))
+        out.append(row)
+    return out, energy, t, D, E, x

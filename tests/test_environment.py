@@ -209,3 +209,22 @@ def test_energy_gradient_matches_finite_differences():
             minus[l1][l2].data[i] -= eps
             fd = (energy(plus)[0] - energy(minus)[0]) / (2 * eps)
             assert abs(g.data[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (l1, l2, i, g.data[i], fd)
+
+
+def test_natural_gradient_is_min_norm_least_squares():
+    """Conjugate gradient from zero on the normal equations converges to the
+    minimum-norm least-squares solution: 2 x = 2 pinv(D) E (numpy lstsq),
+    3x3 lattice, 6 configurations, truncating cut."""
+    from oracle import environment as E, peps, sampling
+
+    L1, L2, N, cut = 3, 3, 4, 3
+    _, _, sites = small_lattice(L1, L2, N, seed=18)
+    cfgs = [c for c in configs(L1, L2, N, 20, seed=19) if peps.psi(sites, c, cut)[0] != 0][:6]
+    H = sampling.tv_hamiltonian(1.0, 1.0)
+    # D has rank <= 5 (6 centred rows): the residual vanishes within 5 steps;
+    # the error rule stops there (iterating on a zero residual divides by ~0,
+    # in the reference as here)
+    _, _, steps, D, Ev, x = E.natural_gradient(sites, cfgs, cut, H, step=50, error=1e-10)
+    assert steps <= 6
+    want = 2 * np.linalg.lstsq(D, Ev, rcond=None)[0]
+    assert np.max(np.abs(x - want)) <= 1e-6 * np.max(np.abs(want))

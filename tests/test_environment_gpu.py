@@ -151,3 +151,29 @@ def test_environment_energy_gradient_matches_oracle():
         for l2 in range(L2):
             same_structure(grad[l1][l2], want[l1][l2])
             assert rel(np.asarray(grad[l1][l2].storage), want[l1][l2].data) < 1e-8, (l1, l2)
+
+
+def test_environment_natural_gradient_matches_oracle():
+    """Natural gradient by conjugate gradient over chains against the oracle
+    (3 steps), 4x4 at a truncating cut."""
+    from oracle import environment as E, peps as opeps, sampling
+    from test_gpu_ops import rel, same_structure
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    L1, L2, N, cut = 4, 4, 8, 3
+    spec, storages, sites = small_lattice(L1, L2, N, seed=48)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    cand = configs(L1, L2, N, 16, seed=49)
+    start = np.stack([c for c in cand if opeps.psi(sites, c, cut)[0] != 0][:4])
+    smp = EnvironmentSampler(eng, start, cut=cut)
+    grad, energy, steps = smp.natural_gradient(step=3)
+    want, e_want, steps_want, _, _, _ = E.natural_gradient(sites, start, cut, sampling.tv_hamiltonian(1.0, 1.0),
+                                                           step=3)
+    assert steps == steps_want == 3
+    assert abs(energy - e_want) <= 1e-9 * max(1.0, abs(e_want))
+    for l1 in range(L1):
+        for l2 in range(L2):
+            same_structure(grad[l1][l2], want[l1][l2])
+            assert rel(np.asarray(grad[l1][l2].storage), want[l1][l2].data) < 1e-7, (l1, l2)

@@ -471,6 +471,62 @@ class EnvironmentSampler:
             out.append(row)
         return out, energy
 
+    def natural_gradient(self, step, error=0.0, configs=None, reweight=None):
+        """The observer's natural gradient by conjugate gradient
+        (observer.py:576-675): solve (D^H D) x = D^H E with
+        D = sqrt(w_s) (Delta_s - <Delta>) and E = conj(sqrt(w_s) (E_s - <E>))
+        over the chains, ``step`` iterations at most, stopping early when
+        |r|^2 / |b|^2 < error^2; the result 2 x goes back to site tensors and
+        through the trivial-metric conjugate.  Returns (gradient, energy,
+        steps taken)."""
+        torch = Bt._torch()
+        holes, psi = self.holes(configs)
+        e = self.local_energy(configs)
+        w = np.ones(len(e)) if reweight is None else np.asarray(reweight, dtype=np.float64).copy()
+        w[(psi == 0) | ~np.isfinite(e)] = 0.0
+        if w.sum() <= 0:
+            raise ValueError("no chain with a non-zero amplitude")
+        w = w / w.sum()
+        energy = float(np.sum(w * np.where(w > 0, e, 0.0)))
+        # Delta (chains x parameters): the holes of all sites side by side
+        blocks, shapes = [], []
+        for l1 in range(self.L1):
+            for l2 in range(self.L2):
+                h = holes[l1][l2]
+                n = int(h.sizes[0])
+                blocks.append(torch.stack([h.bufs[b][o:o + n] for b, o in zip(h.buf_idx, h.offs)]))
+                shapes.append((l1, l2, n))
+        delta = torch.cat(blocks, dim=1)
+        wt = torch.as_tensor(w, dtype=delta.dtype, device=delta.device)
+        mean = (wt[:, None] * delta).sum(0)
+        D = (delta - mean[None, :]) * wt.sqrt()[:, None]
+        E = torch.as_tensor(np.where(w > 0, np.sqrt(w) * (e - energy), 0.0), dtype=delta.dtype, device=delta.device)
+        b = D.T @ E
+        b2 = float(b @ b)
+        x = torch.zeros_like(b)
+        r = b.clone()
+        p = r.clone()
+        r2 = float(r @ r)
+        t = 0
+        while t < step and not (error != 0.0 and b2 > 0 and error ** 2 > r2 / b2):
+            Dp = D @ p
+            alpha = r2 / float(Dp @ Dp)
+            x = x + alpha * p
+            r = r - alpha * (D.T @ Dp)
+            nr2 = float(r @ r)
+            p = r + (nr2 / r2) * p
+            r2 = nr2
+            t += 1
+        x = 2 * x
+        out = [[None] * self.L2 for _ in range(self.L1)]
+        off = 0
+        for l1, l2, n in shapes:
+            g = holes[l1][l2].tensor(0)
+            g._data = x[off:off + n].clone()
+            off += n
+            out[l1][l2] = g.conjugate(trivial_metric=True)
+        return out, energy, t
+
     # -- lanes --------------------------------------------------------------------------
     def _child(self):
         """A view of this sampler for one group of chains: shares the engine,
