@@ -163,6 +163,10 @@ int tns_engine_reset_stats(tns_engine* e);
  * svd_reg8, svd_reg16, svd_reg24; per kernel: launches, total ms,
  * algorithmic work (bytes for permute/fill/diag, flops for the others) */
 int tns_engine_set_profile(tns_engine* e, int on);
+/* the caller changed the site tensors' values in place (same buffers and
+ * block structure), e.g. a gradient step (sampling_lattice/lattice.py:936-948):
+ * the projected sites are rebuilt on the next evaluation */
+int tns_engine_refresh_sites(tns_engine* e);
 int tns_engine_kernel_stats(const tns_engine* e, double* values, int n);
 int tns_engine_free(tns_engine* e);
 

@@ -177,3 +177,47 @@ def test_environment_natural_gradient_matches_oracle():
         for l2 in range(L2):
             same_structure(grad[l1][l2], want[l1][l2])
             assert rel(np.asarray(grad[l1][l2].storage), want[l1][l2].data) < 1e-7, (l1, l2)
+
+
+def test_gradient_step_lowers_exact_energy():
+    """apply_gradient (lattice.py:936-948) with the observer's gradient over
+    every configuration of a 2x3 lattice (weights |psi|^2, i.e. the exact
+    expectation) lowers the exact energy by eta |g|^2 to first order; the
+    engine and the sampler re-project the updated sites."""
+    import itertools
+
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+    from tnsp_amd.sampler import TVSampler
+
+    L1, L2, N = 2, 3, 3
+    spec, storages, _ = small_lattice(L1, L2, N, seed=50)
+    lat = pp.Lattice(spec, storages)
+    eng = AmplitudeEngine(lat, chunk=64, lanes=1)
+    allc = []
+    for occ in itertools.combinations(range(L1 * L2), N):
+        c = np.zeros(L1 * L2, dtype=np.int64)
+        c[list(occ)] = 1
+        allc.append(c.reshape(L1, L2))
+    allc = np.stack(allc)
+    cfgs = allc[eng.evaluate(allc) != 0]
+
+    def exact_energy():
+        obs = TVSampler(eng, cfgs)
+        return float(np.sum(obs.psi ** 2 * obs.local_energy()) / np.sum(obs.psi ** 2))
+
+    e0 = exact_energy()
+    smp = EnvironmentSampler(eng, cfgs, cut=10_000)
+    grad, eg = smp.energy_gradient(reweight=smp.psi ** 2)
+    assert abs(eg - e0) <= 1e-9 * max(1.0, abs(e0))
+    g2 = sum(float((np.asarray(g.storage) ** 2).sum()) for row in grad for g in row)
+    before = np.asarray(lat.sites[0][1].storage).copy()
+    g01 = np.asarray(grad[0][1].transpose(lat.sites[0][1].names).storage)
+    eta = 1e-4 / max(g2, 1e-300) * abs(e0)
+    lat.apply_gradient(grad, eta)
+    smp.refresh_sites()
+    np.testing.assert_allclose(np.asarray(lat.sites[0][1].storage), before - eta * g01, rtol=0, atol=1e-14)
+    e1 = exact_energy()
+    assert e1 < e0
+    assert abs((e0 - e1) - eta * g2) <= 0.1 * eta * g2

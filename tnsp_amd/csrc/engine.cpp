@@ -865,6 +865,12 @@ int tns_engine_reset_stats(tns_engine* e) {
     })
 }
 
+int tns_engine_refresh_sites(tns_engine* e) {
+    TNS_TRY({
+        for (auto& l : e->lanes) l->refresh_sites();
+    })
+}
+
 int tns_engine_set_profile(tns_engine* e, int on) {
     TNS_TRY({
         e->profile = on != 0;

@@ -113,6 +113,8 @@ class Engine {
         timer_.reset();
     }
     void set_profile(bool on) { profile_ = on; }
+    // the raw site buffers changed in place: project again on the next call
+    void refresh_sites() { proj_.clear(); }
     const KernelTimer& timer() const { return timer_; }
 
    private:

@@ -82,6 +82,11 @@ class AmplitudeEngine:
                                                   psi.ctypes.data_as(_lib.c_dblp), ctypes.c_void_p(_stream())))
         return psi
 
+    def refresh_sites(self):
+        """The lattice's site tensors changed in place (``Lattice.apply_gradient``):
+        project them again on the next evaluation."""
+        _lib.check(_lib.lib().tns_engine_refresh_sites(self.handle))
+
     def stats(self):
         vals = (ctypes.c_double * len(STAT_NAMES))()
         _lib.check(_lib.lib().tns_engine_stats(self.handle, vals, len(STAT_NAMES)))

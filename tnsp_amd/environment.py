@@ -95,6 +95,15 @@ class EnvironmentSampler:
         self._hole_pos = {n: i for i, n in enumerate(hole_name_order(self.L1, self.L2))}
         self._site_pos, self._t_pos = _fermion_order_index(self.L1, self.L2)
 
+    def refresh_sites(self):
+        """The lattice changed (``Lattice.apply_gradient``): project the sites
+        again here and in the engine; chains keep their configurations and
+        psi is recomputed."""
+        self.engine.refresh_sites()
+        self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
+                       for l2 in range(self.L2)] for l1 in range(self.L1)]
+        self.psi = np.abs(self.engine.evaluate(self.configs, cut=self.cut))
+
     # -- batches of site tensors --------------------------------------------------
     def _site(self, l1, l2, occ):
         """Projected site (l1, l2) for occupations ``occ`` (one per item)."""

@@ -169,6 +169,22 @@ class Lattice:
             self._projected[key] = t
         return t
 
+    def apply_gradient(self, gradient, step_size):
+        """``SamplingLattice.apply_gradient`` (lattice.py:936-948): every site
+        tensor minus ``step_size`` times its gradient, in place (the same
+        device buffers, so an engine built on this lattice only needs
+        ``refresh_sites``).  Projected sites are recomputed on demand."""
+        for l1 in range(self.spec.L1):
+            for l2 in range(self.spec.L2):
+                site = self.sites[l1][l2]
+                g = gradient[l1][l2]
+                if g.names != site.names:
+                    g = g.transpose(site.names)
+                if g._data.numel() != site._data.numel():
+                    raise ValueError("gradient and site differ in shape")
+                site._data.sub_(g._data.to(site._data.dtype) * float(step_size))
+        self._projected = {k: v for k, v in self._projected.items() if k[0] == "shrinker"}
+
     def shrinker(self, l1, l2, occupation):
         """``_get_shrinker`` (lattice.py:291-318): ``P`` the dimension-one
         edge of the occupation, ``Q`` the conjugate physical edge, one unit
