@@ -58,7 +58,38 @@ def parse():
     ap.add_argument("--sweep-only", action="store_true", help="time only the sweep and observer (one JSON line)")
     ap.add_argument("--sweep-chains", type=int, default=256)
     ap.add_argument("--sweep-count", type=int, default=2, help="timed sweeps after one untimed sweep")
+    ap.add_argument("--vmc", default="", help="run tetragono-style gradient descent on this workload (one JSON line)")
+    ap.add_argument("--vmc-steps", type=int, default=10)
+    ap.add_argument("--vmc-chains", type=int, default=256)
+    ap.add_argument("--vmc-sweeps", type=int, default=1, help="sweeps (samples per chain) per gradient step")
+    ap.add_argument("--vmc-step-size", type=float, default=0.01)
+    ap.add_argument("--vmc-natural", action="store_true", help="natural gradient by conjugate gradient")
     return ap.parse_args()
+
+
+def vmc_run(args, log):
+    """Gradient descent with sweep sampling (tnsp_amd.vmc), inputs resident:
+    per step the sweeps, the observer's energy and holes, the gradient and
+    the update are timed together."""
+    import numpy as np
+
+    from tnsp_amd import peps
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+    from tnsp_amd.vmc import gradient_descent
+
+    spec = getattr(peps, args.vmc)
+    lat = peps.Lattice(spec)
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
+    cand = peps.sample_configurations(spec, 4 * args.vmc_chains, seed=77)
+    start = cand[eng.evaluate(cand) != 0][:args.vmc_chains]
+    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
+    recs = gradient_descent(smp, args.vmc_steps, args.vmc_step_size, sweeps=args.vmc_sweeps,
+                            natural=args.vmc_natural, log=log)
+    return {"workload": args.vmc, "chains": len(start), "lanes": args.lanes, "natural": args.vmc_natural,
+            "step_size": args.vmc_step_size, "sweeps_per_step": args.vmc_sweeps,
+            "energy_per_site": [r["energy_per_site"] for r in recs],
+            "seconds_per_step": float(np.mean([r["seconds"] for r in recs[1:]] or [recs[0]["seconds"]]))}
 
 
 def sweep_timing(args, log):
@@ -280,6 +311,9 @@ def main():
     if args.sweep_only:
         args.sweep = args.sweep or args.workload
         print(json.dumps({"environment_sweep": sweep_timing(args, log)}), flush=True)
+        return
+    if args.vmc:
+        print(json.dumps({"vmc": vmc_run(args, log)}), flush=True)
         return
     spec = getattr(peps, args.workload)
     storages = peps.random_storages(spec)
