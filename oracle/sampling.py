@@ -22,11 +22,6 @@ import numpy as np
 
 from . import peps, tat
 
-FU1 = tat.KINDS["FermiU1"]
-
-
-def _edge(segments, arrow):
-    return tat.Edge(FU1, [((q,), 1) for q in segments], arrow)
 
 
 def rename_io(t, m):
@@ -40,13 +35,25 @@ def rename_io(t, m):
     return t.edge_rename({k: v for k, v in res.items() if k in t.names})
 
 
-def fermi_operators():
-    """common_tensor/Fermi.py:18-37 (real part): CP = c^dagger, CM = c with
-    the charge carried by the one-dimensional ``T`` edge."""
-    EF = _edge([0, 1], False)
-    ET = _edge([0, -1], True)
-    CP = tat.Tensor(FU1, ["O0", "I0", "T"], [EF, ET, _edge([-1], False)]).range_(1, 0)
-    CM = tat.Tensor(FU1, ["O0", "I0", "T"], [EF, ET, _edge([1], True)]).range_(1, 0)
+def fermi_operators(kind_name="FermiU1"):
+    """common_tensor/Fermi.py:18-37 (FermiU1) and common_tensor/Parity.py:
+    22-38 (FermiZ2), real part: CP = c^dagger, CM = c with the charge carried
+    by the one-dimensional ``T`` edge."""
+    K = tat.KINDS[kind_name]
+    if kind_name == "FermiU1":
+        EF = tat.Edge(K, [((0,), 1), ((1,), 1)], False)
+        ET = tat.Edge(K, [((0,), 1), ((-1,), 1)], True)
+        CP = tat.Tensor(K, ["O0", "I0", "T"], [EF, ET, tat.Edge(K, [((-1,), 1)], False)]).range_(1, 0)
+        CM = tat.Tensor(K, ["O0", "I0", "T"], [EF, ET, tat.Edge(K, [((1,), 1)], True)]).range_(1, 0)
+    elif kind_name == "FermiZ2":
+        EF = tat.Edge(K, [((0,), 1), ((1,), 1)], False)
+        ET = tat.Edge(K, [((0,), 1), ((1,), 1)], True)
+        CP = tat.Tensor(K, ["O0", "I0", "T"], [EF, ET, tat.Edge(K, [((1,), 1)], False)])
+        CP[{"O0": ((1,), 0), "I0": ((0,), 0), "T": ((1,), 0)}] = 1.0
+        CM = tat.Tensor(K, ["O0", "I0", "T"], [EF, ET, tat.Edge(K, [((1,), 1)], True)])
+        CM[{"O0": ((0,), 0), "I0": ((1,), 0), "T": ((1,), 0)}] = 1.0
+    else:
+        raise ValueError("fermion operators for FermiU1 and FermiZ2 only")
     C0C1 = rename_io(CP, [0]).contract(rename_io(CM, [1]), {("T", "T")})
     C1C0 = rename_io(CP, [1]).contract(rename_io(CM, [0]), {("T", "T")})
     CC = C0C1 + C1C0
@@ -54,10 +61,10 @@ def fermi_operators():
     return CP, CM, CC, N
 
 
-def tv_hamiltonian(t=1.0, V=1.0):
-    """tetraku/tetraku/models/tV/__init__.py:39-45: H = -t CC + V NN with
-    NN = N_0 (x) N_1."""
-    _, _, CC, N = fermi_operators()
+def tv_hamiltonian(t=1.0, V=1.0, kind_name="FermiU1"):
+    """tetraku/tetraku/models/tV/__init__.py:39-45 (and tV/parity for
+    FermiZ2): H = -t CC + V NN with NN = N_0 (x) N_1."""
+    _, _, CC, N = fermi_operators(kind_name)
     NN = rename_io(N, [0]).contract(rename_io(N, [1]), set())
     return CC * (-t) + NN * V
 
@@ -65,7 +72,8 @@ def tv_hamiltonian(t=1.0, V=1.0):
 def element_pool(H):
     """tensor_element.py:84-111: {in occupations: {out occupations: element
     tensor}}; element tensors have one-element edges (symmetry of the point,
-    the operator's arrows).  Occupations are the U1 charges of the points."""
+    the operator's arrows).  Occupations are the charges of the points
+    (U(1): particle number, Z(2): parity, both the occupation here)."""
     body = H.rank // 2
     result = {}
     for pos_idx in H.block_order:
@@ -76,12 +84,12 @@ def element_pool(H):
             value = block[idx]
             if value == 0:
                 continue
-            edges = [tat.Edge(FU1, [(syms[i], 1)], H.edges[i].arrow) for i in range(H.rank)]
-            tmpl = tat.Tensor(FU1, list(H.names), edges)
+            edges = [tat.Edge(H.kind, [(syms[i], 1)], H.edges[i].arrow) for i in range(H.rank)]
+            tmpl = tat.Tensor(H.kind, list(H.names), edges)
             tmpl.data[:] = value
-            point = {H.names[i]: (syms[i][0], idx[i]) for i in range(H.rank)}
-            edge_in = tuple(-point[f"I{i}"][0] for i in range(body))
-            edge_out = tuple(point[f"O{i}"][0] for i in range(body))
+            point = {H.names[i]: (syms[i], idx[i]) for i in range(H.rank)}
+            edge_in = tuple(H.kind.neg(point[f"I{i}"][0])[0] for i in range(body))
+            edge_out = tuple(point[f"O{i}"][0][0] for i in range(body))
             result.setdefault(edge_in, {})[edge_out] = tmpl
     return result
 

@@ -221,3 +221,49 @@ def test_gradient_step_lowers_exact_energy():
     e1 = exact_energy()
     assert e1 < e0
     assert abs((e0 - e1) - eta * g2) <= 0.1 * eta * g2
+
+
+def small_lattice_z2(L1, L2, N, seed):
+    from oracle import peps
+    from tnsp_amd import peps as pp
+
+    spec = pp.LatticeSpec("FermiZ2", L1, L2, (2, 2), N, 1000, seed=seed)
+    storages = pp.random_storages(spec)
+    sd = {"kind": spec.kind, "L1": L1, "L2": L2, "N": N, "segments": list(spec.segments)}
+    return spec, storages, peps.build_sites(sd, storages)
+
+
+def test_environment_fermi_z2_sweep_energy_holes():
+    """FermiZ2 lattice (even/odd bonds, D = 4), 4x4 at a truncating cut:
+    sweep decisions, observer energy and holes against the oracle."""
+    from oracle import environment as E, peps as opeps, sampling
+    from test_gpu_ops import rel, same_structure
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+
+    L1, L2, N, cut = 4, 4, 8, 4
+    spec, storages, sites = small_lattice_z2(L1, L2, N, seed=60)
+    eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
+    cand = configs(L1, L2, N, 16, seed=61)
+    start = np.stack([c for c in cand if opeps.psi(sites, c, cut)[0] != 0][:3])
+    H = sampling.tv_hamiltonian(1.0, 1.0, kind_name="FermiZ2")
+    smp = EnvironmentSampler(eng, start, cut=cut, seed=62, lanes=2)
+    # observer and holes at the start configurations
+    np.testing.assert_allclose(smp.local_energy(), [E.local_energy(sites, c, cut, H) for c in start],
+                               rtol=1e-9, atol=1e-9)
+    holes, _ = smp.holes()
+    for b, cfg in enumerate(start):
+        want = E.holes(sites, cfg, cut)
+        for l1 in range(L1):
+            for l2 in range(L2):
+                same_structure(holes[l1][l2].tensor(b), want[l1][l2])
+                assert rel(np.asarray(holes[l1][l2].tensor(b).storage), want[l1][l2].data) < 1e-10
+    # one sweep with the hole(()) refresh against the oracle's
+    gen = np.random.Generator(np.random.PCG64(62))
+    smp.sweep()
+    u = np.stack([gen.random(len(start)) for _ in range(len(smp.bonds))])
+    for b in range(len(start)):
+        cfg, _, ws = E.sweep(sites, start[b], cut, u[:, b])
+        assert np.array_equal(smp.configs[b], cfg)
+        assert abs(smp.psi[b] - ws) <= 1e-10 * ws

@@ -151,3 +151,29 @@ def test_autocorrelation_of_ar1():
     tau = integrated_autocorrelation(x)
     exact = (1 + phi) / (1 - phi)
     assert abs(tau - exact) / exact < 0.1
+
+
+def test_local_energy_closed_form_fermi_z2():
+    """FermiZ2 lattices (tV/parity): the closed form with an even T edge
+    equals the observer contraction with the parity operators
+    (common_tensor/Parity.py) on a 3x3 lattice."""
+    from oracle import peps, sampling
+    from tnsp_amd import peps as pp
+    from tnsp_amd.sampler import TVSampler
+
+    spec = pp.LatticeSpec("FermiZ2", 3, 3, (2, 2), 4, 1000, seed=31)
+    storages = pp.random_storages(spec)
+    sd = {"kind": spec.kind, "L1": 3, "L2": 3, "N": 4, "segments": list(spec.segments)}
+    sites = peps.build_sites(sd, storages)
+    rng = np.random.default_rng(32)
+    cfgs = []
+    while len(cfgs) < 3:
+        c = np.zeros(9, dtype=np.int64)
+        c[rng.choice(9, 4, replace=False)] = 1
+        c = c.reshape(3, 3)
+        if peps.psi(sites, c, spec.cut)[0] != 0:
+            cfgs.append(c)
+    H = sampling.tv_hamiltonian(1.0, 1.0, kind_name="FermiZ2")
+    got = TVSampler(OracleAmplitudes(spec, sites), np.stack(cfgs)).local_energy()
+    want = [sampling.local_energy(sites, c, spec.cut, H) for c in cfgs]
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-10)

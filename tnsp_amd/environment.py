@@ -93,6 +93,8 @@ class EnvironmentSampler:
         from .sampler import _fermion_order_index, hole_name_order
 
         self._hole_pos = {n: i for i, n in enumerate(hole_name_order(self.L1, self.L2))}
+        # parity of the T edge at site (0, 0): -N for FermiU1, even for FermiZ2
+        self._t_parity = spec.N % 2 if spec.kind == "FermiU1" else 0
         self._site_pos, self._t_pos = _fermion_order_index(self.L1, self.L2)
 
     def refresh_sites(self):
@@ -282,7 +284,7 @@ class EnvironmentSampler:
         for j, n in enumerate(names):
             pos[j] = self._hole_pos[n]
             if n == "T":
-                odd[:, j] = self.engine.spec.N % 2
+                odd[:, j] = self._t_parity
             elif n.startswith("P_"):
                 _, a, b, _ = n.split("_")
                 odd[:, j] = flat[:, int(a) * self.L2 + int(b)]
@@ -317,7 +319,7 @@ class EnvironmentSampler:
         between = (self._site_pos > lo) & (self._site_pos < hi)
         k = flat[:, between].sum(1)
         if lo < self._t_pos < hi:
-            k = k + self.engine.spec.N
+            k = k + self._t_parity
         return 1 - 2 * (k % 2)
 
     def local_energy(self, configs=None, log=None):

@@ -63,22 +63,28 @@ def _fermion_order_index(L1, L2):
 
 
 class TVSampler:
-    """Lockstep Metropolis sweep sampling of B chains of a FermiU1 PEPS with
-    the tV Hamiltonian; ``engine`` is an :class:`AmplitudeEngine`."""
+    """Lockstep Metropolis sweep sampling of B chains of a FermiU1 or
+    FermiZ2 PEPS with the tV Hamiltonian; ``engine`` is an
+    :class:`AmplitudeEngine`."""
 
     def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0):
         spec = engine.spec
-        if spec.kind != "FermiU1":
-            raise ValueError("the tV sampler needs a FermiU1 lattice")
+        if spec.kind not in ("FermiU1", "FermiZ2"):
+            raise ValueError("the tV sampler needs a FermiU1 or FermiZ2 lattice")
         self.engine = engine
         self.L1, self.L2 = spec.L1, spec.L2
         self.N = spec.N
+        # parity of the T edge: -N for FermiU1, even for FermiZ2
+        self.t_parity = spec.N % 2 if spec.kind == "FermiU1" else 0
         self.t, self.V, self.alpha = float(t), float(V), float(alpha)
         self.configs = np.ascontiguousarray(np.asarray(configs), dtype=np.int8).copy()
         if self.configs.ndim != 3 or self.configs.shape[1:] != (self.L1, self.L2):
             raise ValueError("configs must have shape (chains, L1, L2)")
-        if np.any(self.configs.reshape(len(self.configs), -1).sum(1) != self.N):
+        counts = self.configs.reshape(len(self.configs), -1).sum(1)
+        if spec.kind == "FermiU1" and np.any(counts != self.N):
             raise ValueError("every configuration must hold N particles")
+        if spec.kind == "FermiZ2" and np.any(counts % 2):
+            raise ValueError("every configuration must hold an even number of particles")
         self.gen = np.random.Generator(np.random.PCG64(seed))
         self.bonds = bond_order(self.L1, self.L2)
         self.psi = engine.evaluate(self.configs)
@@ -125,13 +131,13 @@ class TVSampler:
 
     def hop_sign(self, flat, ia, ib):
         """(-1)^k, k = particles strictly between sites ia and ib in the hole
-        edge order (T counts N)."""
+        edge order (T counts its parity: N for FermiU1, even for FermiZ2)."""
         pa, pb = self._pos[ia], self._pos[ib]
         lo, hi = min(pa, pb), max(pa, pb)
         between = (self._pos > lo) & (self._pos < hi)
         k = flat[:, between].sum(1).astype(np.int64)
         if lo < self._tpos < hi:
-            k = k + self.N
+            k = k + self.t_parity
         return 1 - 2 * (k % 2)
 
     def local_energy(self, configs=None, psi=None):
