@@ -179,27 +179,52 @@ class EnvironmentSampler:
             self.psi[dbl[acc]] = wss[acc]
 
     # -- passes -----------------------------------------------------------------------
+    # -- row environments (single_layer_auxiliaries.py:254-312, 437-442) --------------
+    def _down_mps(self, one):
+        """_down_to_up[l1] for l1 = L1 .. 1 from the current rows."""
+        down = {self.L1: [one] * self.L2}
+        for l1 in reversed(range(1, self.L1)):
+            down[l1] = self._two_line("DULR", down[l1 + 1], self._row(l1))
+        return down
+
+    def _right_tailed(self, row, up, dn, one):
+        """_inline_right_to_left_tailed[l1, c] for c = L2 .. 0 (:283-312);
+        ``up`` = _up_to_down[l1-1], ``dn`` = _down_to_up[l1+1]."""
+        rt, tailed = {}, None
+        for c in reversed(range(self.L2 + 1)):
+            if c == self.L2:
+                right = one
+            else:
+                right = Bt.contract(tailed, row[c].rename({"L": "L2"}), [("L2", "R"), ("U", "D")])
+                right = Bt.contract(right, up[c].rename({"L": "L1"}), [("L1", "R"), ("U", "D")])
+            tailed = right if c == 0 else Bt.contract(right, dn[c - 1], [("L3", "R")]).rename({"L": "L3"})
+            rt[c] = tailed
+        return rt
+
+    @staticmethod
+    def _left_start(one, up):
+        """_inline_left_to_right_tailed[l1, -1] (:254-282)."""
+        return Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
+
+    @staticmethod
+    def _left_extend(lt, site, dn_c, up_next):
+        """_inline_left_to_right_tailed[l1, c] from the one at c - 1 and the
+        site at column c (``dn_c`` = _down_to_up_site[l1+1, c], ``up_next`` =
+        _up_to_down_site[l1-1, c+1])."""
+        left = Bt.contract(lt, site.rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
+        left = Bt.contract(left, dn_c.rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
+        return Bt.contract(left, up_next, [("R1", "L")]).rename({"R": "R1"})
+
     def _horizontal(self, log=None, observe=False):
         L1, L2 = self.L1, self.L2
         B = len(self.configs)
         one = self._ones(B)
-        down = {L1: [one] * L2}
-        for l1 in reversed(range(1, L1)):
-            down[l1] = self._two_line("DULR", down[l1 + 1], self._row(l1))
+        down = self._down_mps(one)
         up = [one] * L2
         for l1 in range(L1):
             dn = down[l1 + 1]
             row = self._row(l1)
-            # _inline_right_to_left_tailed[l1, c] for c = L2 .. 0 (:283-312)
-            rt, tailed = {}, None
-            for c in reversed(range(L2 + 1)):
-                if c == L2:
-                    right = one
-                else:
-                    right = Bt.contract(tailed, row[c].rename({"L": "L2"}), [("L2", "R"), ("U", "D")])
-                    right = Bt.contract(right, up[c].rename({"L": "L1"}), [("L1", "R"), ("U", "D")])
-                tailed = right if c == 0 else Bt.contract(right, dn[c - 1], [("L3", "R")]).rename({"L": "L3"})
-                rt[c] = tailed
+            rt = self._right_tailed(row, up, dn, one)
             if l1 == 0:
                 # hole(()) = _inline_right_to_left[0, 0]: psi at the start of the sweep
                 if observe:
@@ -207,8 +232,7 @@ class EnvironmentSampler:
                 else:
                     self.psi = np.abs(rt[0].values())
                     self._hole_cache = self.psi.copy()
-            # _inline_left_to_right_tailed[l1, -1] (:254-282)
-            lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
+            lt = self._left_start(one, up)
             for l2 in range(L2 - 1):
                 def replace(idx, t0, t1, lt=lt, l2=l2):
                     # single_layer_auxiliaries.py:641-651
@@ -220,10 +244,8 @@ class EnvironmentSampler:
                                                                     ("R3", "L3")])
 
                 (self._observe if observe else self._term)((l1, l2), (l1, l2 + 1), replace)
-                site = self._site(l1, l2, self.configs[:, l1, l2])
-                left = Bt.contract(lt, site.rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
-                left = Bt.contract(left, dn[l2].rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
-                lt = Bt.contract(left, up[l2 + 1], [("R1", "L")]).rename({"R": "R1"})
+                # the column as decided
+                lt = self._left_extend(lt, self._site(l1, l2, self.configs[:, l1, l2]), dn[l2], up[l2 + 1])
             if l1 < L1 - 1:
                 up = self._two_line("UDLR", up, self._row(l1))
             if log:
@@ -378,34 +400,22 @@ class EnvironmentSampler:
         L1, L2 = self.L1, self.L2
         B = len(self.configs)
         one = self._ones(B)
-        down = {L1: [one] * L2}
-        for l1 in reversed(range(1, L1)):
-            down[l1] = self._two_line("DULR", down[l1 + 1], self._row(l1))
+        down = self._down_mps(one)
         up = [one] * L2
         raw = [[None] * L2 for _ in range(L1)]
         ws = None
         for l1 in range(L1):
             dn = down[l1 + 1]
             row = self._row(l1)
-            rt, tailed = {}, None
-            for c in reversed(range(L2 + 1)):
-                if c == L2:
-                    right = one
-                else:
-                    right = Bt.contract(tailed, row[c].rename({"L": "L2"}), [("L2", "R"), ("U", "D")])
-                    right = Bt.contract(right, up[c].rename({"L": "L1"}), [("L1", "R"), ("U", "D")])
-                tailed = right if c == 0 else Bt.contract(right, dn[c - 1], [("L3", "R")]).rename({"L": "L3"})
-                rt[c] = tailed
+            rt = self._right_tailed(row, up, dn, one)
             if l1 == 0:
                 ws = rt[0]
-            lt = Bt.contract(one, up[0], [("R1", "L")]).rename({"R": "R1"})
+            lt = self._left_start(one, up)
             for l2 in range(L2):
                 big = Bt.contract(lt, rt[l2 + 1], [("R1", "L1"), ("R3", "L3")])
                 raw[l1][l2] = big.rename({"R2": "R0", "L2": "L0", "U": "U0", "D": "D0"})
                 if l2 < L2 - 1:
-                    left = Bt.contract(lt, row[l2].rename({"R": "R2"}), [("R2", "L"), ("D", "U")])
-                    left = Bt.contract(left, dn[l2].rename({"R": "R3"}), [("R3", "L"), ("D", "U")])
-                    lt = Bt.contract(left, up[l2 + 1], [("R1", "L")]).rename({"R": "R1"})
+                    lt = self._left_extend(lt, row[l2], dn[l2], up[l2 + 1])
             if l1 < L1 - 1:
                 up = self._two_line("UDLR", up, row)
         w = ws.values()
