@@ -53,11 +53,12 @@ def parse():
     ap.add_argument("--ess-epochs", type=int, default=6)
     ap.add_argument("--ess-learned-k", default="",
                     help="proposals per learned sample, comma-separated (default: bonds / 2)")
-    ap.add_argument("--sweep", default="", help="also time tetragono's sweep (cached-environment replace) and "
+    ap.add_argument("--sweep", default="C4", help="also time tetragono's sweep (cached-environment replace) and "
                                                 "its observer energy on this workload")
     ap.add_argument("--sweep-only", action="store_true", help="time only the sweep and observer (one JSON line)")
     ap.add_argument("--sweep-chains", type=int, default=256)
-    ap.add_argument("--sweep-count", type=int, default=2, help="timed sweeps after one untimed sweep")
+    ap.add_argument("--sweep-count", type=int, default=1, help="timed sweeps after one untimed sweep")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the sweep timing of the default run")
     ap.add_argument("--vmc", default="", help="run tetragono-style gradient descent on this workload (one JSON line)")
     ap.add_argument("--vmc-steps", type=int, default=10)
     ap.add_argument("--vmc-chains", type=int, default=256)
@@ -92,10 +93,11 @@ def vmc_run(args, log):
             "seconds_per_step": float(np.mean([r["seconds"] for r in recs[1:]] or [recs[0]["seconds"]]))}
 
 
-def sweep_timing(args, log):
+def sweep_timing(args, log, eng=None):
     """Time EnvironmentSampler sweeps (tetragono's SweepSampling with
     replace over cached environments) and one observer pass over all
-    chains, inputs resident, on ``args.sweep``."""
+    chains, inputs resident, on ``args.sweep`` (reusing ``eng`` when it
+    holds that workload's PEPS)."""
     import numpy as np
     import torch
 
@@ -104,7 +106,8 @@ def sweep_timing(args, log):
     from tnsp_amd.environment import EnvironmentSampler
 
     spec = getattr(peps, args.sweep)
-    eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.lanes)
+    if eng is None:
+        eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.lanes)
     cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
     smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
@@ -377,7 +380,14 @@ def main():
     ess = None
     if args.ess:
         ess = ess_comparison(args, log)
-    env_sweep = sweep_timing(args, log) if args.sweep else None
+    env_sweep = None
+    if args.sweep and not args.no_sweep:
+        # outside the timed region; reported, never fatal to the bench line
+        try:
+            env_sweep = sweep_timing(args, log, eng if args.sweep == args.workload and args.precision == "fp64"
+                                     else None)
+        except Exception as e:  # noqa: BLE001
+            env_sweep = {"error": f"{type(e).__name__}: {e}"}
     line = {
         "metric": "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec",
         "value": value,
