@@ -1,0 +1,89 @@
+"""Oracle of tetragono's direct sampling (oracle/direct.py).
+
+Pins, on small lattices whose every configuration can be listed:
+* with no truncation (single- and double-layer cuts above every bond
+  dimension) and alpha = 1, the probability of drawing each configuration is
+  |psi(s)|^2 / sum_s' |psi(s')|^2, with psi from the pinned amplitude
+  (oracle.peps.psi); FermiU1 at fixed particle number and FermiZ2 at even
+  parity -- a dropped conjugate, a wrong fermion sign in the double layer or a
+  wrong block of the site hole breaks this;
+* at truncating cuts the draw is still a normalised sequential distribution:
+  the probabilities of all configurations add up to one, and a draw driven by
+  uniforms reports the probability that the forced draw of its configuration
+  gives;
+* ``choice`` is tetragono's running subtraction (first index where it goes
+  negative, the last index when rounding leaves it positive).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from test_sampler import small_lattice
+
+
+def z2_lattice(L1, L2, seed):
+    from oracle import peps
+    from tnsp_amd import peps as pp
+
+    spec = pp.LatticeSpec("FermiZ2", L1, L2, (2, 1), 2, 1000, seed=seed)
+    storages = pp.random_storages(spec)
+    sd = {"kind": "FermiZ2", "L1": L1, "L2": L2, "N": 2, "segments": list(spec.segments)}
+    return spec, storages, peps.build_sites(sd, storages)
+
+
+def all_configs(L1, L2, kind, N):
+    for flat in itertools.product([0, 1], repeat=L1 * L2):
+        if (kind == "FermiU1" and sum(flat) == N) or (kind == "FermiZ2" and sum(flat) % 2 == 0):
+            yield np.array(flat, dtype=np.int64).reshape(L1, L2)
+
+
+@pytest.mark.parametrize("kind", ["FermiU1", "FermiZ2"])
+def test_uncut_probability_is_born_rule(kind):
+    from oracle import direct as Dr, peps
+
+    L1, L2, N = 2, 3, 3
+    if kind == "FermiU1":
+        _, _, sites = small_lattice(L1, L2, N, seed=3)
+    else:
+        _, _, sites = z2_lattice(L1, L2, seed=5)
+    big = 10_000
+    down = Dr.double_layer_down_to_up(sites, big)
+    p, w = [], []
+    for cfg in all_configs(L1, L2, kind, N):
+        pr, _, rhos = Dr.direct_sample(sites, big, big, forced=cfg, down=down)
+        p.append(pr)
+        w.append(peps.psi(sites, cfg, big)[0] ** 2)
+        assert all(abs(r.sum() - 1) < 1e-12 for r in rhos)
+    p, w = np.array(p), np.array(w)
+    assert np.all(w > 0)
+    assert np.allclose(p, w / w.sum(), rtol=1e-9, atol=1e-15)
+
+
+def test_truncated_draw_is_normalised_and_consistent():
+    from oracle import direct as Dr
+
+    L1, L2 = 2, 3
+    _, _, sites = z2_lattice(L1, L2, seed=7)
+    cut, cut2 = 2, 2
+    down = Dr.double_layer_down_to_up(sites, cut2)
+    total = sum(Dr.direct_sample(sites, cut, cut2, forced=cfg, down=down)[0]
+                for cfg in all_configs(L1, L2, "FermiZ2", 0))
+    assert abs(total - 1) < 1e-12
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        u = rng.random(L1 * L2)
+        p, cfg, _ = Dr.direct_sample(sites, cut, cut2, uniforms=u, down=down)
+        pf, cf, _ = Dr.direct_sample(sites, cut, cut2, forced=cfg, down=down)
+        assert np.array_equal(cfg, cf)
+        assert p == pytest.approx(pf, rel=1e-13)
+
+
+def test_choice():
+    from oracle.direct import choice
+
+    rho = np.array([0.25, 0.5, 0.25])
+    assert choice(0.1, rho) == 0
+    assert choice(0.3, rho) == 1
+    assert choice(0.76, rho) == 2
+    assert choice(1.0, rho) == 2  # rounding: never beyond the last index
