@@ -65,6 +65,10 @@ def parse():
     ap.add_argument("--vmc-sweeps", type=int, default=1, help="sweeps (samples per chain) per gradient step")
     ap.add_argument("--vmc-step-size", type=float, default=0.01)
     ap.add_argument("--vmc-natural", action="store_true", help="natural gradient by conjugate gradient")
+    ap.add_argument("--vmc-direct", type=int, default=0, help="direct samples per gradient step (0: sweep sampling)")
+    ap.add_argument("--direct", default="", help="time tetragono's direct sampling on this workload (one JSON line)")
+    ap.add_argument("--direct-samples", type=int, default=256)
+    ap.add_argument("--direct-cut2", type=int, default=4, help="double-layer bond dimension of direct sampling")
     return ap.parse_args()
 
 
@@ -85,12 +89,52 @@ def vmc_run(args, log):
     cand = peps.sample_configurations(spec, 4 * args.vmc_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.vmc_chains]
     smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
+    direct = None
+    if args.vmc_direct:
+        from tnsp_amd.direct import DirectSampler
+
+        direct = DirectSampler(lat, cut2=args.direct_cut2, seed=2)
     recs = gradient_descent(smp, args.vmc_steps, args.vmc_step_size, sweeps=args.vmc_sweeps,
-                            natural=args.vmc_natural, log=log)
+                            natural=args.vmc_natural, log=log, direct=direct, direct_samples=args.vmc_direct)
     return {"workload": args.vmc, "chains": len(start), "lanes": args.lanes, "natural": args.vmc_natural,
             "step_size": args.vmc_step_size, "sweeps_per_step": args.vmc_sweeps,
+            "direct_samples_per_step": args.vmc_direct, "direct_cut2": args.direct_cut2 if direct else None,
+            "instability": [r.get("instability") for r in recs],
             "energy_per_site": [r["energy_per_site"] for r in recs],
             "seconds_per_step": float(np.mean([r["seconds"] for r in recs[1:]] or [recs[0]["seconds"]]))}
+
+
+def direct_timing(args, log):
+    """Time DirectSampler draws (tetragono's DirectSampling, the samples in
+    lockstep) on ``args.direct``, inputs resident: the double layer once
+    (reported apart), one untimed draw of a few samples, then one timed draw
+    of ``args.direct_samples``; the weights |psi|^2 / p use the engine's psi."""
+    import numpy as np
+    import torch
+
+    from tnsp_amd import peps
+    from tnsp_amd.direct import DirectSampler, instability, reweight
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec = getattr(peps, args.direct)
+    lat = peps.Lattice(spec)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    dsm = DirectSampler(lat, cut2=args.direct_cut2, seed=1)
+    torch.cuda.synchronize()
+    t_double = time.time() - t0
+    log(f"direct: double layer built in {t_double:.2f}s")
+    dsm.sample(min(8, args.direct_samples), log=lambda m: log(f"direct (warm-up): {m}"))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    poss, cfg = dsm.sample(args.direct_samples, log=lambda m: log(f"direct: {m}"))
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
+    w = reweight(eng.evaluate(cfg), poss)
+    return {"workload": args.direct, "samples": args.direct_samples, "cut": spec.cut, "cut2": args.direct_cut2,
+            "double_layer_s": t_double, "draw_s": dt, "samples_per_s": args.direct_samples / dt,
+            "redrawn": dsm.redrawn, "zero_psi": int(np.sum(w == 0)), "instability": float(instability(w))}
 
 
 def sweep_timing(args, log, eng=None):
@@ -317,6 +361,9 @@ def main():
         return
     if args.vmc:
         print(json.dumps({"vmc": vmc_run(args, log)}), flush=True)
+        return
+    if args.direct:
+        print(json.dumps({"direct_sampling": direct_timing(args, log)}), flush=True)
         return
     spec = getattr(peps, args.workload)
     storages = peps.random_storages(spec)

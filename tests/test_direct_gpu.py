@@ -61,3 +61,29 @@ def test_direct_uncut_born_rule():
     want = np.array([w[c.astype(np.int64).tobytes()] / Z for c in cfg])
     np.testing.assert_allclose(poss, want, rtol=1e-9, atol=0)
     assert len({c.tobytes() for c in cfg}) > 1
+
+
+def test_direct_weights_and_gradient_steps_uncut():
+    """Without truncation every direct sample weighs |psi|^2 / p = Z
+    (instability zero); the gradient-descent driver with direct sampling
+    rebuilds the double layer after each update, so the weights stay equal
+    for the updated lattice too."""
+    from tnsp_amd import peps as pp
+    from tnsp_amd.direct import DirectSampler, instability, reweight
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.environment import EnvironmentSampler
+    from tnsp_amd.vmc import gradient_descent
+
+    L1, L2, N = 2, 3, 3
+    spec, storages, _ = small_lattice(L1, L2, N, seed=50)
+    lat = pp.Lattice(spec, storages)
+    eng = AmplitudeEngine(lat, chunk=64, lanes=1)
+    big = 10_000
+    dsm = DirectSampler(lat, cut=big, cut2=big, seed=4)
+    poss, cfg = dsm.sample(16)
+    w = reweight(eng.evaluate(cfg, cut=big), poss)
+    assert np.all(w > 0) and instability(w) < 1e-7
+    smp = EnvironmentSampler(eng, cfg, cut=big)
+    recs = gradient_descent(smp, 2, 1e-3, direct=dsm, direct_samples=16)
+    print(recs, flush=True)
+    assert all(r["instability"] < 1e-7 and np.isfinite(r["energy_per_site"]) for r in recs)

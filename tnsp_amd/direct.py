@@ -34,6 +34,27 @@ from . import batch as Bt
 from .peps import _one
 
 
+def reweight(psi, possibility):
+    """The observer's weight of a direct sample (observer.py:333-342):
+    |psi(s)|^2 / p(s), zero where psi(s) is zero (the observer counts such a
+    sample and adds no weight)."""
+    psi = np.abs(np.asarray(psi, dtype=np.float64))
+    p = np.asarray(possibility, dtype=np.float64)
+    out = np.zeros(len(psi))
+    nz = psi != 0
+    out[nz] = psi[nz] ** 2 / p[nz]
+    return out
+
+
+def instability(weights):
+    """observer.py:417-431: standard deviation over mean of the weights of
+    all samples (zero weights included in the count)."""
+    w = np.asarray(weights, dtype=np.float64)
+    mean = w.sum() / len(w)
+    var = (w * w).sum() / len(w) - mean * mean
+    return (np.sqrt(var) if var > 0 else 0.0) / mean
+
+
 def _contract_all(a, b, pairs):
     """safe_contract with ``contract_all_physics_edges`` (utility.py:
     340-350): every common name starting with ``P`` is contracted too."""
@@ -191,7 +212,7 @@ class DirectSampler:
         return np.maximum(np.array(rows), 0.0), np.array(occ)
 
     # -- sampling (sampling.py:276-360) -----------------------------------------------------
-    def _draw(self, B, uniforms):
+    def _draw(self, B, uniforms, log=None):
         L1, L2 = self.L1, self.L2
         cfg = np.zeros((B, L1, L2), dtype=np.int8)
         poss = np.ones(B)
@@ -232,9 +253,11 @@ class DirectSampler:
                 left = self._three_two_line("LR", left, (n0[l2], n1[l2], t2[l2], c1[l2], c0[l2]))
             row = [Bt.TensorBatch.select(self._proj[l1][l2], cfg[:, l1, l2]) for l2 in range(L2)]
             up = self._single_two_line(up, row)
+            if log:
+                log(f"row {l1} of {L1} drawn for {B} samples")
         return poss, cfg, alive
 
-    def sample(self, count, uniforms=None, max_redraw=20):
+    def sample(self, count, uniforms=None, max_redraw=20, log=None):
         """``count`` independent draws: (possibility, configurations).  The
         uniforms (count x L1*L2, site by site in row-major order) default to
         this sampler's generator; draws that hit a site with no weight are
@@ -245,13 +268,13 @@ class DirectSampler:
         uniforms = np.asarray(uniforms, dtype=np.float64)
         if uniforms.shape != (count, sites):
             raise ValueError("uniforms must have shape (count, L1*L2)")
-        poss, cfg, alive = self._draw(count, uniforms)
+        poss, cfg, alive = self._draw(count, uniforms, log)
         todo = np.nonzero(~alive)[0]
         for _ in range(max_redraw):
             if not len(todo):
                 break
             self.redrawn += len(todo)
-            p2, c2, a2 = self._draw(len(todo), self.gen.random((len(todo), sites)))
+            p2, c2, a2 = self._draw(len(todo), self.gen.random((len(todo), sites)), log)
             poss[todo], cfg[todo] = p2, c2
             todo = todo[~a2]
         if len(todo):

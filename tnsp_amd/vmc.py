@@ -15,28 +15,39 @@ import numpy as np
 
 
 def gradient_descent(sampler, grad_steps, step_size, sweeps=1, natural=False, cg_step=20, cg_error=1e-4,
-                     log=None):
+                     log=None, direct=None, direct_samples=0):
     """Runs ``grad_steps`` steps; returns a list of per-step records
-    (energy per site before the step, seconds, samples)."""
+    (energy per site before the step, seconds, samples, and for direct
+    sampling the weights' instability)."""
+    from .direct import instability, reweight
+
     lat = sampler.lattice
     sites = lat.spec.L1 * lat.spec.L2
     out = []
     for k in range(grad_steps):
         t0 = time.time()
-        samples = []
-        for _ in range(sweeps):
-            sampler.sweep()
-            samples.append(sampler.configs.copy())
-        cfg = np.concatenate(samples)
-        if natural:
-            grad, energy, steps = sampler.natural_gradient(cg_step, cg_error, configs=cfg)
+        w, extra = None, {}
+        if direct is not None:
+            poss, cfg = direct.sample(direct_samples)
+            w = reweight(sampler.engine.evaluate(cfg, cut=sampler.cut), poss)
+            extra["instability"] = float(instability(w))
         else:
-            grad, energy = sampler.energy_gradient(configs=cfg)
+            samples = []
+            for _ in range(sweeps):
+                sampler.sweep()
+                samples.append(sampler.configs.copy())
+            cfg = np.concatenate(samples)
+        if natural:
+            grad, energy, steps = sampler.natural_gradient(cg_step, cg_error, configs=cfg, reweight=w)
+        else:
+            grad, energy = sampler.energy_gradient(configs=cfg, reweight=w)
             steps = 0
         lat.apply_gradient(grad, step_size)
         sampler.refresh_sites()
+        if direct is not None:
+            direct.refresh_all()
         rec = {"step": k, "energy_per_site": energy / sites, "seconds": time.time() - t0, "samples": len(cfg),
-               "cg_steps": steps}
+               "cg_steps": steps, **extra}
         out.append(rec)
         if log:
             log(f"gradient step {k}: energy/site {rec['energy_per_site']:.6f}, {rec['seconds']:.1f}s")
