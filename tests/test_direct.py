@@ -12,7 +12,9 @@ Pins, on small lattices whose every configuration can be listed:
   uniforms reports the probability that the forced draw of its configuration
   gives;
 * ``choice`` is tetragono's running subtraction (first index where it goes
-  negative, the last index when rounding leaves it positive).
+  negative, the last index when rounding leaves it positive);
+* the observer's sample weight and instability (tnsp_amd.direct) against
+  their closed forms.
 """
 import itertools
 
@@ -87,3 +89,15 @@ def test_choice():
     assert choice(0.3, rho) == 1
     assert choice(0.76, rho) == 2
     assert choice(1.0, rho) == 2  # rounding: never beyond the last index
+
+
+def test_reweight_and_instability():
+    """observer.py:333-342, 417-431: weight |psi|^2 / p, none for psi = 0
+    (counted all the same); instability = population standard deviation
+    over the mean of all weights."""
+    from tnsp_amd.direct import instability, reweight
+
+    w = reweight([0.0, 2.0, -1.0], [0.5, 0.25, 0.5])
+    assert np.array_equal(w, [0.0, 16.0, 2.0])
+    assert instability(w) == pytest.approx(np.std(w) / np.mean(w), rel=1e-12)
+    assert instability([3.0, 3.0, 3.0]) == 0.0
