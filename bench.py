@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--vmc-direct", type=int, default=0, help="direct samples per gradient step (0: sweep sampling)")
     ap.add_argument("--direct", default="", help="time tetragono's direct sampling on this workload (one JSON line)")
     ap.add_argument("--direct-samples", type=int, default=256)
+    ap.add_argument("--direct-default", default="C2", help="workload of the direct-sampling timing of the default run")
+    ap.add_argument("--no-direct", action="store_true", help="skip the direct-sampling timing of the default run")
     ap.add_argument("--direct-cut2", type=int, default=4, help="double-layer bond dimension of direct sampling")
     return ap.parse_args()
 
@@ -435,6 +437,14 @@ def main():
                                      else None)
         except Exception as e:  # noqa: BLE001
             env_sweep = {"error": f"{type(e).__name__}: {e}"}
+    direct = None
+    if not args.no_direct:
+        # direct sampling on the 4x4 workload, also outside the timed region
+        try:
+            args.direct = args.direct_default
+            direct = direct_timing(args, log)
+        except Exception as e:  # noqa: BLE001
+            direct = {"error": f"{type(e).__name__}: {e}"}
     line = {
         "metric": "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec",
         "value": value,
@@ -464,7 +474,7 @@ def main():
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
                   "plain_path_configs_per_s": plain,
                   "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned, "ess": ess,
-                  "environment_sweep": env_sweep},
+                  "environment_sweep": env_sweep, "direct_sampling": direct},
     }
     print(json.dumps(line), flush=True)
 
