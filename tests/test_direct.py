@@ -101,3 +101,54 @@ def test_reweight_and_instability():
     assert np.array_equal(w, [0.0, 16.0, 2.0])
     assert instability(w) == pytest.approx(np.std(w) / np.mean(w), rel=1e-12)
     assert instability([3.0, 3.0, 3.0]) == 0.0
+
+
+@pytest.mark.parametrize("kind_name,order", [("FermiU1", ["I0", "O0"]), ("FermiU1", ["O0", "I0"]),
+                                             ("FermiZ2", ["O0", "I0"])])
+def test_site_weight_readout(kind_name, order):
+    """The GPU sampler's host readout of the site holes (DirectSampler._rho)
+    gives the oracle's weights (edge rename, transpose to (I, O), diagonal of
+    each block, clipped) for holes stored in either edge order; blocks of two
+    states check the diagonal indexing, the (O0, I0) order the sign of odd
+    blocks.  Storage is handed over as host tensors: only the block layout of
+    the native shape planner and the readout run."""
+    from types import SimpleNamespace
+
+    import torch
+
+    from oracle import direct as Dr, tat
+    from tnsp_amd.direct import DirectSampler
+    from tnsp_amd.shape import KINDS, Edge, Shape
+
+    segs = [(0, 2), (1, 3)]
+    kind, okind = KINDS[kind_name], tat.KINDS[kind_name]
+    phys = Edge(kind, segs, kind_name == "FermiU1")
+    edges = {"O0": phys, "I0": phys.conjugate()}
+    shape = Shape.create(kind, order, [edges[n] for n in order])
+    oedges = {"O0": tat.Edge(okind, segs, kind_name == "FermiU1")}
+    oedges["I0"] = oedges["O0"].conjugate()
+    rng = np.random.default_rng(3)
+    n, size = 3, shape.size
+    data = rng.normal(size=(n, size))
+    class Hole:
+        names = list(order)
+        model = SimpleNamespace(kind=kind)
+        sizes = np.full(n, size)
+        bufs = [torch.from_numpy(data.reshape(-1).copy())]
+        buf_idx = np.zeros(n, dtype=np.int64)
+        offs = np.arange(n) * size
+
+        def __len__(self):
+            return n
+
+        def tensor(self, i):
+            return SimpleNamespace(names=shape.names, edges=shape.edges, _shape=shape)
+
+    hole = Hole()
+    rho, occ = DirectSampler._rho(DirectSampler.__new__(DirectSampler), hole)
+    assert list(occ) == [0, 0, 1, 1, 1]
+    for i in range(n):
+        t = tat.Tensor(okind, list(order), [oedges[nm] for nm in order])
+        t.data[:] = data[i]
+        want, _ = Dr.site_rho(t)
+        np.testing.assert_array_equal(rho[i], want)

@@ -178,38 +178,51 @@ class DirectSampler:
         r = _contract_all(r, c0.rename({"U": "U2"}), [("U2", "D"), ("R", "L"), ("T", "T")])
         return Bt.contract(r, right[4].rename({"U": "U3"}), [("U3", "D"), ("R", "L")])
 
+    @staticmethod
+    def _layout(t, kind, swap):
+        """For one site hole: the storage index and sign of each diagonal
+        entry (index -1 and sign 0 where the contraction left the block
+        empty: it weighs nothing) and the occupation each entry stands for."""
+        edges = dict(zip(t.names, t.edges))
+        eo, ei = edges["O0"], edges["I0"]
+        blocks = {p: off for p, off, _ in t._shape.blocks()}
+        idx, sgn, occ = [], [], []
+        for q, d in eo.segments:
+            pi, po = ei.position_by_symmetry(kind.neg(q)), eo.position_by_symmetry(q)
+            off = blocks.get((pi, po) if t.names[0] == "I0" else (po, pi))
+            s = -1.0 if swap and kind.parity(q) else 1.0
+            for j in range(d):
+                # entry (j, j) of a row-major d x d block
+                idx.append(-1 if off is None else off + j * (d + 1))
+                sgn.append(0.0 if off is None else s)
+                occ.append(int(np.asarray(q).reshape(-1)[0]))
+        return np.array(idx, dtype=np.int64), np.array(sgn), np.array(occ)
+
     def _rho(self, hole):
         """sampling.py:318-331 for every sample: the diagonal block of each
         segment of the ket edge ``O0``, in that edge's segment order, read
         in the order (I0, O0) (TAT's transpose of two odd edges changes the
         sign of their odd blocks), negative rounding clipped.  Returns
-        (weights, occupation of each column)."""
+        (weights, occupation of each column).  Every sample's hole has the
+        same two edges (the site's physical edge and its conjugate), so one
+        layout serves all items of equal size; the weights of all samples
+        come to the host in one copy."""
         if sorted(hole.names) != ["I0", "O0"]:
             raise RuntimeError(f"site hole has edges {hole.names}, expected I0 and O0")
         swap = hole.names == ["O0", "I0"]
-        host = {b: hole.bufs[b].double().cpu().numpy() for b in np.unique(hole.buf_idx)}
         kind = hole.model.kind
-        rows, occ = [], None
-        for i in range(len(hole)):
-            t = hole.tensor(i)
-            edges = dict(zip(t.names, t.edges))
-            eo, ei = edges["O0"], edges["I0"]
-            vals = []
-            blocks = {p: off for p, off, _ in t._shape.blocks()}
-            for q, d in eo.segments:
-                pi = ei.position_by_symmetry(kind.neg(q))
-                po = eo.position_by_symmetry(q)
-                off = blocks.get((pi, po) if t.names[0] == "I0" else (po, pi))
-                if off is None:  # a sector the contraction left empty weighs nothing
-                    vals.extend([0.0] * d)
-                    continue
-                blk = host[hole.buf_idx[i]][hole.offs[i] + off:hole.offs[i] + off + d * d].reshape(d, d)
-                sign = -1.0 if swap and kind.parity(q) else 1.0
-                vals.extend(sign * np.diagonal(blk))
-            rows.append(vals)
-            if occ is None:
-                occ = [int(np.asarray(q).reshape(-1)[0]) for q, d in eo.segments for _ in range(d)]
-        return np.maximum(np.array(rows), 0.0), np.array(occ)
+        idx0, sgn0, occ = self._layout(hole.tensor(0), kind, swap)
+        rows = np.zeros((len(hole), len(idx0)))
+        for b in np.unique(hole.buf_idx):
+            host = hole.bufs[b].double().cpu().numpy()
+            sel = np.nonzero(hole.buf_idx == b)[0]
+            same = sel[hole.sizes[sel] == hole.sizes[0]]
+            at = np.minimum(hole.offs[same][:, None] + np.maximum(idx0, 0)[None, :], len(host) - 1)
+            rows[same] = host[at] * sgn0
+            for i in sel[hole.sizes[sel] != hole.sizes[0]]:
+                idx, sgn, _ = self._layout(hole.tensor(i), kind, swap)
+                rows[i] = host[np.minimum(hole.offs[i] + np.maximum(idx, 0), len(host) - 1)] * sgn
+        return np.maximum(rows, 0.0), occ
 
     # -- sampling (sampling.py:276-360) -----------------------------------------------------
     def _draw(self, B, uniforms, log=None):
