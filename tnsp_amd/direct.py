@@ -281,6 +281,8 @@ class DirectSampler:
         uniforms = np.asarray(uniforms, dtype=np.float64)
         if uniforms.shape != (count, sites):
             raise ValueError("uniforms must have shape (count, L1*L2)")
+        if count == 0:
+            return np.zeros(0), np.zeros((0, self.L1, self.L2), dtype=np.int8)
         poss, cfg, alive = self._draw(count, uniforms, log)
         todo = np.nonzero(~alive)[0]
         for _ in range(max_redraw):
