@@ -14,7 +14,8 @@ Pins, on small lattices whose every configuration can be listed:
 * ``choice`` is tetragono's running subtraction (first index where it goes
   negative, the last index when rounding leaves it positive);
 * the observer's sample weight and instability (tnsp_amd.direct) against
-  their closed forms.
+  their closed forms; the GPU sampler's host readout of the site holes
+  against the oracle's, and its redraw loop with scripted draws.
 """
 import itertools
 
@@ -152,3 +153,47 @@ def test_site_weight_readout(kind_name, order):
         t.data[:] = data[i]
         want, _ = Dr.site_rho(t)
         np.testing.assert_array_equal(rho[i], want)
+
+
+def _stub_sampler(script):
+    """A DirectSampler whose draws come from ``script`` (a list of
+    (possibility, configuration, alive) per call); no device is touched."""
+    from tnsp_amd.direct import DirectSampler
+
+    smp = DirectSampler.__new__(DirectSampler)
+    smp.L1, smp.L2 = 1, 2
+    smp.gen = np.random.Generator(np.random.PCG64(0))
+    smp.redrawn = 0
+    calls = []
+
+    def draw(B, uniforms, log=None):
+        assert uniforms.shape == (B, 2)
+        p, c, a = script[len(calls)]
+        calls.append(B)
+        return np.array(p, dtype=float)[:B], np.array(c, dtype=np.int8)[:B], np.array(a)[:B]
+
+    smp._draw = draw
+    return smp, calls
+
+
+def test_sample_redraws_dead_and_restricted_draws():
+    """Draws with no weight at some site and draws the restriction rejects
+    are drawn again (only those), as DirectSampling.__call__ restarts them;
+    the count of redraws is kept."""
+    smp, calls = _stub_sampler([
+        ([0.5, 0.0, 0.25], [[[1, 0]], [[0, 1]], [[1, 1]]], [True, False, True]),
+        ([0.5, 0.125], [[[0, 1]], [[1, 0]]], [True, True]),
+    ])
+    poss, cfg = smp.sample(3, restrict=lambda c: c.reshape(len(c), -1).sum(axis=1) == 1)
+    assert calls == [3, 2]
+    assert smp.redrawn == 2
+    assert np.array_equal(poss, [0.5, 0.5, 0.125])
+    assert np.array_equal(cfg.reshape(3, -1), [[1, 0], [0, 1], [1, 0]])
+
+
+def test_sample_gives_up_after_max_redraw():
+    smp, calls = _stub_sampler([([0.0], [[[0, 0]]], [False])] * 3)
+    with pytest.raises(RuntimeError, match="no accepted draw"):
+        smp.sample(1, max_redraw=2)
+    assert calls == [1, 1, 1]
+    assert len(smp.sample(0)[0]) == 0

@@ -270,11 +270,13 @@ class DirectSampler:
                 log(f"row {l1} of {L1} drawn for {B} samples")
         return poss, cfg, alive
 
-    def sample(self, count, uniforms=None, max_redraw=20, log=None):
+    def sample(self, count, uniforms=None, max_redraw=20, log=None, restrict=None):
         """``count`` independent draws: (possibility, configurations).  The
         uniforms (count x L1*L2, site by site in row-major order) default to
-        this sampler's generator; draws that hit a site with no weight are
-        redrawn from fresh uniforms."""
+        this sampler's generator; draws that hit a site with no weight, and
+        draws that ``restrict`` (configurations -> bool per draw, tetragono's
+        ``restrict_subspace``, sampling.py:352-355) rejects, are drawn again
+        from fresh uniforms, at most ``max_redraw`` times."""
         sites = self.L1 * self.L2
         if uniforms is None:
             uniforms = self.gen.random((count, sites))
@@ -283,15 +285,21 @@ class DirectSampler:
             raise ValueError("uniforms must have shape (count, L1*L2)")
         if count == 0:
             return np.zeros(0), np.zeros((0, self.L1, self.L2), dtype=np.int8)
+
+        def good(c, a):
+            if restrict is None:
+                return a
+            return a & np.asarray(restrict(c), dtype=bool)
+
         poss, cfg, alive = self._draw(count, uniforms, log)
-        todo = np.nonzero(~alive)[0]
+        todo = np.nonzero(~good(cfg, alive))[0]
         for _ in range(max_redraw):
             if not len(todo):
                 break
             self.redrawn += len(todo)
             p2, c2, a2 = self._draw(len(todo), self.gen.random((len(todo), sites)), log)
             poss[todo], cfg[todo] = p2, c2
-            todo = todo[~a2]
+            todo = todo[~good(c2, a2)]
         if len(todo):
-            raise RuntimeError(f"{len(todo)} direct samples found no weight in {max_redraw} redraws")
+            raise RuntimeError(f"{len(todo)} direct samples found no accepted draw in {max_redraw} redraws")
         return poss, cfg
