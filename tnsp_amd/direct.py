@@ -90,12 +90,14 @@ class DirectSampler:
                        for l2 in range(self.L2)] for l1 in range(self.L1)]
         ones = [Bt.TensorBatch.repeat(self._one, 1)] * self.L2
         down = {self.L1: ones}
+        # (ket, bra) of every raw site, one item each: shared by the double
+        # layer and by every draw's row before it is shrunk
+        self._kb = [[None] * self.L2 for _ in range(self.L1)]
         for l1 in reversed(range(self.L1)):
-            row = []
             for l2 in range(self.L2):
                 ket = Bt.TensorBatch.repeat(self.lattice.sites[l1][l2], 1)
-                row.append((ket, Bt.conjugate(ket)))
-            down[l1] = self._double_two_line("DULR", down[l1 + 1], row)
+                self._kb[l1][l2] = (ket, Bt.conjugate(ket))
+            down[l1] = self._double_two_line("DULR", down[l1 + 1], self._kb[l1])
         self._down = down
 
     def _canonical(self, stage, left, right, left0, left1, right0, right1, cut, normalize):
@@ -236,9 +238,10 @@ class DirectSampler:
         for l1 in range(L1):
             n0 = up
             c0 = [Bt.conjugate(t) for t in n0]
-            n1 = [Bt.TensorBatch.repeat(self.lattice.sites[l1][l2], B) for l2 in range(L2)]
-            c1 = [Bt.conjugate(t) for t in n1]
-            t2 = [d.take(np.zeros(B, dtype=np.int64)) for d in self._down[l1 + 1]]
+            every = np.zeros(B, dtype=np.int64)
+            n1 = [self._kb[l1][l2][0].take(every) for l2 in range(L2)]
+            c1 = [self._kb[l1][l2][1].take(every) for l2 in range(L2)]
+            t2 = [d.take(every) for d in self._down[l1 + 1]]
             right = {L2: ones}
             for c in reversed(range(L2)):
                 right[c] = self._three_two_line("RL", right[c + 1], (n0[c], n1[c], t2[c], c1[c], c0[c]))
