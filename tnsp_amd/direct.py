@@ -263,12 +263,15 @@ class DirectSampler:
                 k += 1
                 poss *= np.where(alive, np.nan_to_num(rho[np.arange(B), pick]), 0.0)
                 cfg[:, l1, l2] = occ[pick]
+                if l2 == L2 - 1:
+                    break  # nothing reads the row's environment past its last site
                 shr = Bt.TensorBatch.select(self._shr[l1][l2], cfg[:, l1, l2]).rename({"P": "P0"})
                 n1[l2] = Bt.contract(n1[l2], shr, [("P0", "Q")])
                 c1[l2] = Bt.conjugate(n1[l2])
                 left = self._three_two_line("LR", left, (n0[l2], n1[l2], t2[l2], c1[l2], c0[l2]))
-            row = [Bt.TensorBatch.select(self._proj[l1][l2], cfg[:, l1, l2]) for l2 in range(L2)]
-            up = self._single_two_line(up, row)
+            if l1 < L1 - 1:  # the boundary MPS of the drawn rows, for the next row
+                row = [Bt.TensorBatch.select(self._proj[l1][l2], cfg[:, l1, l2]) for l2 in range(L2)]
+                up = self._single_two_line(up, row)
             if log:
                 log(f"row {l1} of {L1} drawn for {B} samples")
         return poss, cfg, alive
