@@ -7,6 +7,7 @@ Pins, on small lattices whose every configuration can be listed:
   (oracle.peps.psi); FermiU1 at fixed particle number and FermiZ2 at even
   parity -- a dropped conjugate, a wrong fermion sign in the double layer or a
   wrong block of the site hole breaks this;
+* the double layer alone (uncut, unnormalised) contracts to sum |psi|^2;
 * at truncating cuts the draw is still a normalised sequential distribution:
   the probabilities of all configurations add up to one, and a draw driven by
   uniforms reports the probability that the forced draw of its configuration
@@ -197,3 +198,25 @@ def test_sample_gives_up_after_max_redraw():
         smp.sample(1, max_redraw=2)
     assert calls == [1, 1, 1]
     assert len(smp.sample(0)[0]) == 0
+
+
+def test_double_layer_norm_uncut():
+    """The double-layer boundary MPS of every row (no truncation, no
+    normalisation) contracted along the row is <psi|psi> = sum_s psi(s)^2:
+    the bra layer's conjugation and the traced physical and T edges carry
+    the right fermion signs."""
+    from oracle import direct as Dr, peps
+
+    L1, L2, N = 2, 3, 3
+    _, _, sites = small_lattice(L1, L2, N, seed=13)
+    big = 10_000
+    mps = [peps.one(sites[0][0].kind)] * L2
+    for l1 in reversed(range(L1)):
+        row = [(s, s.conjugate()) for s in sites[l1]]
+        mps = Dr.double_layer_two_line("DULR", mps, row, big, normalize=False)
+    acc = mps[0]
+    for t in mps[1:]:
+        acc = acc.contract(t, {("R", "L")})
+    assert acc.names == []
+    z = sum(peps.psi(sites, cfg, big)[0] ** 2 for cfg in all_configs(L1, L2, "FermiU1", N))
+    assert float(acc.data[0]) == pytest.approx(z, rel=1e-10)
