@@ -43,6 +43,12 @@ typedef struct tns_plan tns_plan;
 const char* tns_last_error(void);
 const char* tns_version(void);
 int tns_device_count(int* count);
+/* Jacobi sweep limit of the SVD kernels (default 60, range 1..1000).  A
+ * sector that has not converged within the limit makes the SVD fail with an
+ * error, as a LAPACK gesvd failure (info > 0) makes TAT raise
+ * (TAT/include/TAT/implement/svd.hpp:125-139 with TAT_ERROR_BITS); lowering
+ * the limit forces that path in tests.  Process-wide. */
+int tns_set_svd_max_sweeps(int sweeps);
 
 /* shapes (TAT::Tensor(names, edges): block layout of core.hpp:152-191) */
 int tns_shape_create(int kind, int rank, const int32_t* names, const int32_t* nseg, const int32_t* syms,

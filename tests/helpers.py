@@ -97,6 +97,14 @@ def load_golden_8x8():
     return json.load(open(os.path.join(here, "peps8x8_amplitudes.json")))
 
 
+def load_golden_zero():
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return json.load(open(os.path.join(here, "zero_amplitudes.json")))
+
+
 def to_gpu(t, dtype="float64"):
     """Copy an oracle tensor (structure + storage) into a GPU tensor."""
     from tnsp_amd import TAT

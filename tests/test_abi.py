@@ -37,3 +37,11 @@ def test_error_path_reports_message():
     assert b"duplicated" in L.tns_last_error()
     rc = L.tns_shape_create(42, 0, names, nseg, syms, dims, None, ctypes.byref(h))
     assert rc != 0 and b"kind" in L.tns_last_error()
+
+
+def test_svd_sweep_limit_range_checked():
+    from tnsp_amd import _lib
+
+    L = _lib.lib()
+    assert L.tns_set_svd_max_sweeps(0) != 0 and b"sweep" in L.tns_last_error()
+    assert L.tns_set_svd_max_sweeps(60) == 0

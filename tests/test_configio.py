@@ -38,3 +38,24 @@ def test_bad_files_rejected():
             configio.read_configurations(path)
     with pytest.raises(ValueError):
         configio.occupations(np.array([[[[-2]]]]))
+
+
+def test_trailing_data_of_a_longer_earlier_file_ignored():
+    """tetragono's writer does not truncate: a file first written with 5
+    configurations and then rewritten with 3 keeps the old tail; the reader
+    takes only the 3 the header announces."""
+    from tnsp_amd import configio
+
+    rng = np.random.default_rng(1)
+    first, second = rng.integers(0, 2, size=(5, 3, 4)), rng.integers(0, 2, size=(3, 3, 4))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c.bin")
+        configio.write_configurations(first, path)
+        size = os.path.getsize(path)
+        head = np.array([3, 3, 3, 4, 1], dtype="<i8").tobytes() + second.astype("<i8").tobytes()
+        with open(path, "r+b") as f:  # overwrite in place, as MPI.File.Open(WRONLY | CREATE) does
+            f.write(head)
+        assert os.path.getsize(path) == size
+        back = configio.read_configurations(path)
+        assert back.shape == (3, 3, 4, 1)
+        assert np.array_equal(configio.occupations(back), second)

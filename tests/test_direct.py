@@ -132,6 +132,9 @@ def test_site_weight_readout(kind_name, order):
     rng = np.random.default_rng(3)
     n, size = 3, shape.size
     data = rng.normal(size=(n, size))
+    # item 1 has its own handle of the same structure: it shares the layout
+    shapes = [shape, Shape.create(kind, order, [edges[nm] for nm in order]), shape]
+
     class Hole:
         names = list(order)
         model = SimpleNamespace(kind=kind)
@@ -140,11 +143,15 @@ def test_site_weight_readout(kind_name, order):
         buf_idx = np.zeros(n, dtype=np.int64)
         offs = np.arange(n) * size
 
+        @property
+        def handles(self):
+            return np.array([s.handle.value for s in shapes], dtype=np.uint64)
+
         def __len__(self):
             return n
 
         def tensor(self, i):
-            return SimpleNamespace(names=shape.names, edges=shape.edges, _shape=shape)
+            return SimpleNamespace(names=shapes[i].names, edges=shapes[i].edges, _shape=shapes[i])
 
     hole = Hole()
     rho, occ = DirectSampler._rho(DirectSampler.__new__(DirectSampler), hole)
@@ -154,6 +161,14 @@ def test_site_weight_readout(kind_name, order):
         t.data[:] = data[i]
         want, _ = Dr.site_rho(t)
         np.testing.assert_array_equal(rho[i], want)
+    # an item of the same size whose blocks lie elsewhere (segments in the
+    # other order) is not read through item 0's layout
+    rev = Edge(kind, segs[::-1], kind_name == "FermiU1")
+    redges = {"O0": rev, "I0": rev.conjugate()}
+    shapes[1] = Shape.create(kind, order, [redges[nm] for nm in order])
+    assert shapes[1].size == size
+    with pytest.raises(RuntimeError):
+        DirectSampler._rho(DirectSampler.__new__(DirectSampler), hole)
 
 
 def _stub_sampler(script):

@@ -63,3 +63,61 @@ def test_autotune_keeps_psi():
     assert len(table) == 3 and all(v in (0, 1, 2) for v in table)
     assert ms.shape == (3, 3) and np.all(ms >= 0)
     np.testing.assert_allclose(eng.evaluate(configs), ref, rtol=1e-10)
+
+
+# Per-mode bounds on |d log psi| against the float64 engine at the 8x8
+# workloads (sign agreement is required as well).  The largest deviations
+# measured over 512 configurations of the 8x8 D=8 Dc=64 FermiU1 PEPS were
+# 1.3e-3 (fp32), 6.3e-4 (bf16x6) and 0.22 (bf16x3); on the fixed golden
+# configurations below they are far smaller, and the bounds are ~10x what
+# these configurations give.
+BOUND8 = {"fp32": 1e-3, "bf16x6": 1e-3, "bf16x3": 5e-2}
+MEDIAN8 = {"fp32": 1e-4, "bf16x6": 1e-4, "bf16x3": 1e-2}
+
+
+def golden8(name):
+    from helpers import load_golden_8x8, load_golden_zero
+    from tnsp_amd import peps as ppeps
+
+    amps = {**load_golden_8x8(), **load_golden_zero()}
+    case = amps[name]
+    sd = case["spec"]
+    spec = ppeps.LatticeSpec(sd["kind"], sd["L1"], sd["L2"], tuple(sd["segments"]), sd["N"], sd["cut"], sd["seed"])
+    return spec, ppeps.random_storages(spec), np.array(case["configs"]), np.array([p["value"] for p in case["psi"]])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6", "bf16x3"])
+def test_reduced_precision_gemms_c4(precision):
+    """The 8x8 D=8 Dc=64 FermiU1 workload: TAT's six golden amplitudes and
+    the exact zeros of the zero fixture, each GEMM precision in turn."""
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, ref = golden8("C4_8x8_U1_D8_Dc64")
+    _, _, zconf, zref = golden8("C4_8x8_U1_D8_Dc64_zero")
+    configs = np.concatenate([configs, zconf])
+    ref = np.concatenate([ref, zref])
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages), chunk=8, lanes=2, precision=precision)
+    got = eng.evaluate(configs)
+    zero = ref == 0.0
+    assert np.array_equal(got == 0.0, zero), (precision, got, ref)
+    assert np.array_equal(np.sign(got), np.sign(ref))
+    d = np.abs(np.log(np.abs(got[~zero])) - np.log(np.abs(ref[~zero])))
+    print(precision, "max |dlogpsi|", d.max(), "median", np.median(d))
+    assert d.max() < BOUND8[precision], (precision, d)
+    assert np.median(d) < MEDIAN8[precision], (precision, d)
+
+
+def test_fp32_gemms_c3():
+    """The 8x8 D=6 Dc=36 FermiZ2 workload with its contraction GEMMs in fp32
+    (QR, SVD and storage fp64) against TAT's golden amplitudes."""
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, ref = golden8("C3_8x8_Z2_D6_Dc36")
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages), chunk=4, lanes=2, precision="fp32")
+    got = eng.evaluate(configs)
+    assert np.array_equal(np.sign(got), np.sign(ref))
+    d = np.abs(np.log(np.abs(got)) - np.log(np.abs(ref)))
+    print("C3 fp32 max |dlogpsi|", d.max(), "median", np.median(d))
+    assert d.max() < BOUND8["fp32"], d

@@ -12,10 +12,11 @@ import itertools
 import numpy as np
 import pytest
 
-from helpers import assert_structure, c1_pair, digest, load_golden, load_golden_8x8, storage, to_gpu
+from helpers import assert_structure, c1_pair, digest, load_golden, load_golden_8x8, load_golden_zero, storage, to_gpu
 
 OPS, ARRAYS, AMPS = load_golden()
 AMPS8 = load_golden_8x8()
+ZERO = load_golden_zero()
 KINDS = ["FermiU1", "BoseU1", "FermiZ2"]
 
 
@@ -73,6 +74,19 @@ def test_c1_contract_svd_qr(kind, backend):
     _, s2, _ = C.svd({"j"}, "cu", "cv", "su", "sv", 5)
     assert_structure(s2, ref["S2"])
     assert rel(storage(s2), ARRAYS[f"{kind}_S2"]) < tol
+    # relative cut (PyTAT.hpp:1040-1047: 0 < cut < 1 keeps the values above
+    # cut * the largest one, svd.hpp:430-481)
+    u3, s3, v3 = C.svd({"i", "j"}, "cu", "cv", "su", "sv", 0.3)
+    assert_structure(u3, ref["U3"])
+    assert_structure(s3, ref["S3"])
+    assert_structure(v3, ref["V3"])
+    assert rel(storage(s3), ARRAYS[f"{kind}_S3"]) < tol
+    usv3 = u3.contract(s3, {("cu", "su")}).contract(v3, {("sv", "cv")})
+    assert_structure(usv3, ref["USV3"])
+    assert rel(storage(usv3), ARRAYS[f"{kind}_USV3"]) < tol
+    _, s4, _ = C.svd({"m"}, "cu", "cv", "su", "sv", 0.6)
+    assert_structure(s4, ref["S4"])
+    assert rel(storage(s4), ARRAYS[f"{kind}_S4"]) < tol
     for key, direction in (("QR", "r"), ("QR2", "q")):
         q, r = C.qr(direction, {"m", "n"}, "cq", "cr")
         if key == "QR":
@@ -86,7 +100,7 @@ def test_c1_contract_svd_qr(kind, backend):
 def lattice_case(name):
     from tnsp_amd import peps as ppeps
 
-    case = AMPS[name] if name in AMPS else AMPS8[name]
+    case = AMPS[name] if name in AMPS else (AMPS8[name] if name in AMPS8 else ZERO[name])
     sd = case["spec"]
     spec = ppeps.LatticeSpec(sd["kind"], sd["L1"], sd["L2"], tuple(sd["segments"]), sd["N"], sd["cut"], sd["seed"])
     storages = ppeps.random_storages(spec)
@@ -127,4 +141,56 @@ def test_amplitude_gpu_plain_and_engine_vs_tat(name):
         eng = AmplitudeEngine(lat, chunk=chunk, lanes=lanes)
         got = eng.evaluate(configs)
         np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-300)
+        assert np.array_equal(np.sign(got), np.sign(ref))
+
+
+def test_zero_fixture_has_zero_and_nonzero_amplitudes():
+    """The C4 fixture holds configurations whose TAT amplitude is exactly 0
+    and others whose amplitude is not; the C3 draws had none."""
+    c4 = ZERO["C4_8x8_U1_D8_Dc64_zero"]
+    vals = [p["value"] for p in c4["psi"]]
+    assert c4["zero_count"] > 0 and 0.0 in vals and any(v != 0.0 for v in vals)
+
+
+def test_zero_amplitude_oracle_vs_tat():
+    """The oracle gives exactly 0 where TAT does (one zero and one nonzero
+    C4 configuration: each takes seconds in numpy)."""
+    from oracle import peps
+
+    spec, storages, configs, psis = lattice_case("C4_8x8_U1_D8_Dc64_zero")
+    sd = {"kind": spec.kind, "L1": spec.L1, "L2": spec.L2, "N": spec.N, "segments": list(spec.segments)}
+    sites = peps.build_sites(sd, storages)
+    zi = next(i for i, p in enumerate(psis) if p["value"] == 0.0)
+    ni = next(i for i, p in enumerate(psis) if p["value"] != 0.0)
+    for i in (zi, ni):
+        v, t = peps.psi(sites, configs[i], spec.cut)
+        assert t.names == psis[i]["names"]
+        if psis[i]["value"] == 0.0:
+            assert v == 0.0
+        else:
+            assert abs(v - psis[i]["value"]) <= 1e-10 * abs(psis[i]["value"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(ZERO))
+def test_zero_amplitudes_gpu_plain_and_engine_vs_tat(name):
+    """Exactly zero where TAT's amplitude is exactly zero, TAT's value to
+    1e-10 with its sign elsewhere, on the plain path and on the engine."""
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, psis = lattice_case(name)
+    lat = ppeps.Lattice(spec, storages)
+    ref = np.array([p["value"] for p in psis])
+    zero = ref == 0.0
+    for c in range(len(configs)):
+        if c > 0 and zero[c] == zero[c - 1]:
+            continue  # the plain path is slow: one zero and one nonzero case suffice
+        v, t = ppeps.amplitude_plain(lat, configs[c])
+        assert t.names == psis[c]["names"]
+        assert (v == 0.0) if zero[c] else abs(v - ref[c]) <= 1e-10 * abs(ref[c])
+    for chunk, lanes in ((8, 1), (3, 2)):
+        got = AmplitudeEngine(lat, chunk=chunk, lanes=lanes).evaluate(configs)
+        assert np.array_equal(got == 0.0, zero), (got, ref)
+        np.testing.assert_allclose(got[~zero], ref[~zero], rtol=1e-10, atol=0)
         assert np.array_equal(np.sign(got), np.sign(ref))

@@ -219,3 +219,27 @@ def test_contract_every_gemm_tile(dtype):
     gC = to_gpu(A, dtype).contract(to_gpu(B, dtype), {("k", "k")})
     same_structure(gC, C)
     assert rel(storage(gC), C.data) < (1e-12 if dtype == "float64" else 1e-5)
+
+
+@pytest.mark.gpu
+def test_svd_not_converged_raises():
+    """With the Jacobi sweep limit forced down to one sweep, the random C1
+    sectors do not converge: the kernels mark them and the SVD raises (as
+    TAT does when gesvd reports info > 0); with the limit restored the same
+    SVD succeeds.  Both the LDS kernel (small sectors) and the per-op path
+    are exercised."""
+    from oracle import tat
+    from helpers import c1_pair, to_gpu
+    from tnsp_amd import _lib
+
+    A, B = c1_pair(tat, "FermiU1", 11)
+    C = to_gpu(A).contract(to_gpu(B), {("k", "k"), ("l", "l")})
+    L = _lib.lib()
+    assert L.tns_set_svd_max_sweeps(1) == 0
+    try:
+        with pytest.raises(RuntimeError, match="converge"):
+            C.svd({"i", "j"}, "cu", "cv", "su", "sv", 16)
+    finally:
+        assert L.tns_set_svd_max_sweeps(60) == 0
+    u, s, v = C.svd({"i", "j"}, "cu", "cv", "su", "sv", 16)
+    assert np.all(np.isfinite(np.asarray(s.storage)))

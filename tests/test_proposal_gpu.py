@@ -2,6 +2,7 @@
 learned Metropolis-Hastings sampler's stationary distribution."""
 import numpy as np
 import pytest
+import torch
 
 pytestmark = pytest.mark.gpu
 
@@ -135,3 +136,46 @@ def test_learned_sampler_samples_psi_squared():
     assert s.accepted > 0.05 * s.proposals
     emp = counts / counts.sum()
     assert 0.5 * np.abs(emp - p).sum() < 0.04
+
+
+def test_learned_sampler_acceptance_rule_c4():
+    """Each Metropolis-Hastings decision on the 8x8 D=8 Dc=64 PEPS follows
+    min(1, |psi(s')/psi(s)|^2 q(s|s') / q(s'|s)), with psi(s') recomputed by
+    a separate engine and both proposal probabilities rescored by the
+    network (the reverse hop's and the forward hop's log q)."""
+    from tnsp_amd import peps as pp
+    from tnsp_amd.engine import AmplitudeEngine
+    from tnsp_amd.proposal import HopProposalNet, LearnedSampler, apply_hops
+
+    spec = pp.C4
+    lat = pp.Lattice(spec)
+    eng = AmplitudeEngine(lat, chunk=64, lanes=2)
+    cand = pp.sample_configurations(spec, 96, seed=31)
+    start = cand[eng.evaluate(cand) != 0][:48]
+    net = HopProposalNet(spec.L1, spec.L2, channels=32, layers=4, seed=6)
+    s = LearnedSampler(eng, net, start, seed=8)
+    check = AmplitudeEngine(lat, chunk=16, lanes=1)
+    moved = 0
+    for step in range(3):
+        before, psi_before = s.configs.copy(), s.psi.copy()
+        s.step()
+        rec = s.last
+        ok = rec["valid"]
+        assert ok.all()  # half filling: every configuration has a valid hop
+        hops = rec["hops"]
+        new, rev = apply_hops(before, hops, spec.L1, spec.L2)
+        assert np.array_equal(new, rec["proposed"])
+        psi_new = check.evaluate(new)
+        np.testing.assert_allclose(rec["psi_new"], psi_new, rtol=1e-10, atol=0)
+        lf = net.log_prob(before, torch.as_tensor(hops.astype(np.int32))).cpu().numpy().astype(np.float64)
+        lr = net.log_prob(new, torch.as_tensor(rev.astype(np.int32))).cpu().numpy().astype(np.float64)
+        nz = psi_new != 0
+        expect = 2 * (np.log(np.abs(psi_new[nz])) - np.log(np.abs(psi_before[nz]))) + lr[nz] - lf[nz]
+        np.testing.assert_allclose(rec["log_a"][nz], expect, rtol=0, atol=1e-5)
+        acc = nz & (np.log(np.maximum(rec["uniforms"], 1e-300)) < rec["log_a"])
+        assert np.array_equal(acc, rec["accepted"])
+        after = np.where(acc[:, None, None], new, before)
+        assert np.array_equal(s.configs, after)
+        np.testing.assert_array_equal(s.psi, np.where(acc, psi_new, psi_before))
+        moved += int(acc.sum())
+    assert moved > 0

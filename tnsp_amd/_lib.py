@@ -24,6 +24,7 @@ c_vpp = ctypes.POINTER(ctypes.c_void_p)
 # (name, argtypes); every function returns int status unless noted
 _SIGNATURES = {
     "tns_device_count": [ctypes.POINTER(ctypes.c_int)],
+    "tns_set_svd_max_sweeps": [ctypes.c_int],
     "tns_shape_create": [ctypes.c_int, ctypes.c_int, c_i32p, c_i32p, c_i32p, c_i64p, c_u8p, c_vpp],
     "tns_shape_free": [c_vp],
     "tns_shape_counts": [c_vp, c_i32p, c_i32p, c_i32p, c_i64p, c_i64p],

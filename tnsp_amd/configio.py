@@ -37,9 +37,14 @@ def read_configurations(path):
         raise ValueError("not a configuration file")
     shape = tuple(int(x) for x in raw[2:2 + ndim])
     body = raw[2 + ndim:]
-    if body.size != count * int(np.prod(shape)):
-        raise ValueError("configuration file size does not match its header")
-    return body.reshape((count,) + shape).astype(np.int64)
+    need = count * int(np.prod(shape))
+    # the reference writer opens without truncation (MPI.File.Open with
+    # WRONLY | CREATE), so a file rewritten with fewer configurations keeps
+    # trailing bytes; like the reader (utility.py:402-418) read only what
+    # the header announces
+    if body.size < need:
+        raise ValueError("configuration file is shorter than its header announces")
+    return body[:need].reshape((count,) + shape).astype(np.int64)
 
 
 def occupations(configs):

@@ -192,6 +192,8 @@ extern "C" {
 const char* tns_last_error(void) { return g_error.c_str(); }
 const char* tns_version(void) { return "tnsgpu 0.1 (gfx950)"; }
 
+int tns_set_svd_max_sweeps(int sweeps) { TNS_TRY({ tns::set_svd_max_sweeps(sweeps); }) }
+
 int tns_device_count(int* count) {
     TNS_TRY({
         int c = 0;

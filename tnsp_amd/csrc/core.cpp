@@ -847,7 +847,14 @@ void plan_svd_truncate(SvdPlan& p, const double* svals, int64_t remain_cut, doub
     double smax = 0;
     for (int i = 0; i < ns; i++) {
         total_dim += p.s_k[i];
-        for (int64_t j = 0; j < p.s_k[i]; j++) smax = std::max(smax, svals[p.s_off[i] + j]);
+        for (int64_t j = 0; j < p.s_k[i]; j++) {
+            const double v = svals[p.s_off[i] + j];
+            // the Jacobi kernels mark a sector whose iteration did not
+            // converge by NaN singular values (LAPACK's gesvd reports info > 0
+            // there and TAT raises)
+            if (std::isnan(v)) throw Error("svd: the Jacobi iteration of a sector did not converge");
+            smax = std::max(smax, v);
+        }
     }
     const double threshold = relative_cut > 0 ? relative_cut * smax : 0.0;
     const int64_t rc = (remain_cut < 0 || remain_cut > total_dim) ? total_dim : remain_cut;

@@ -785,7 +785,8 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
     constexpr int RPT = 4;
     const bool cached = mx <= RPT * G;
-    for (int sweep = 0; sweep < 60 && kx > 1; sweep++) {
+    bool converged = kx <= 1;
+    for (int sweep = 0; sweep < tk.max_sweeps && kx > 1; sweep++) {
         if (tid == 0) *flag = 0;
         __syncthreads();
         for (int round = 0; round < P - 1; round++) {
@@ -861,7 +862,16 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
         }
         const int any = *flag;
         __syncthreads();
-        if (!any) break;
+        if (!any) {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged) {
+        // not converged within the sweep limit: NaN singular values tell the
+        // host (the truncation raises on them)
+        for (int c = tid; c < kx; c += SVD_WG) tk.s[c] = __builtin_nan("");
+        return;
     }
     for (int c = g; c < kx; c += NG) {
         double* xc = X + int64_t(c) * mx;
@@ -992,7 +1002,8 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
         xb[i] = (act && r < mx && cb < kx) ? double(trans ? in[int64_t(cb) * n + r] : in[int64_t(r) * n + cb]) : 0.0;
     }
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
-    for (int sweep = 0; sweep < 60 && kx > 1; sweep++) {
+    bool converged = kx <= 1;
+    for (int sweep = 0; sweep < tk.max_sweeps && kx > 1; sweep++) {
         int rot = 0;
         for (int round = 0; round < 2 * h - 1; round++) {
             if (act) {
@@ -1070,7 +1081,14 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
                 cb = ncb;
             }
         }
-        if (!__syncthreads_or(rot)) break;
+        if (!__syncthreads_or(rot)) {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged) {
+        for (int c = tid; c < kx; c += SVR_NT) tk.s[c] = __builtin_nan("");
+        return;
     }
     // sigma, normalised U columns into scratch
     double* X = tk.work;

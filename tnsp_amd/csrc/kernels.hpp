@@ -52,8 +52,13 @@ struct SvdDev {
     double* s;  // singular values are always stored in double
     void* v;
     double* work;  // global scratch (mx*kx + kx*kx + kx doubles) when not LDS-resident
-    int32_t m, n, pad0, pad1;
+    int32_t m, n;
+    int32_t max_sweeps;  // Jacobi sweep limit; a sector not converged by then gets NaN singular values
+    int32_t pad1;
 };
+// default Jacobi sweep limit (the host can lower it to force the
+// non-convergence path in tests)
+constexpr int SVD_MAX_SWEEPS = 60;
 
 // dtype: 0 = float64, 1 = float32
 // Element chunk per workgroup of the permute kernel.

@@ -2,10 +2,22 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <stdexcept>
 
 namespace tns {
+
+namespace {
+std::atomic<int> g_svd_max_sweeps{SVD_MAX_SWEEPS};
+}
+
+int svd_max_sweeps() { return g_svd_max_sweeps.load(std::memory_order_relaxed); }
+
+void set_svd_max_sweeps(int n) {
+    if (n < 1 || n > 1000) throw Error("svd sweep limit must be in [1, 1000]");
+    g_svd_max_sweeps.store(n, std::memory_order_relaxed);
+}
 
 void check_hip(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
@@ -367,6 +379,7 @@ void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, d
         s.work = cur;
         s.m = t.m;
         s.n = t.n;
+        s.max_sweeps = svd_max_sweeps();
         const int64_t mx = std::max(t.m, t.n), kx = std::min(t.m, t.n);
         cur += svd_doubles(t.m, t.n);
         const double fl = 14.0 * double(mx) * double(kx) * double(kx) + 8.0 * double(kx) * double(kx) * double(kx);

@@ -76,6 +76,11 @@ struct KernelTimer {
     ~KernelTimer();
 };
 
+// Jacobi sweep limit written into every SVD task (SVD_MAX_SWEEPS unless
+// lowered, e.g. by a test of the non-convergence path)
+int svd_max_sweeps();
+void set_svd_max_sweeps(int n);
+
 class Batch {
    public:
     explicit Batch(int dtype) : dtype_(dtype) {}

@@ -28,10 +28,26 @@ samples in one launch per kernel kind (``tnsp_amd.batch``).  A sample whose
 weights all vanish at some site is drawn again, as the reference restarts
 such a draw.
 """
+import ctypes
+
 import numpy as np
 
+from . import _lib
 from . import batch as Bt
 from .peps import _one
+
+
+def _block_key(handle):
+    """Block positions and offsets of a native shape handle, as bytes: equal
+    keys mean the same block list in the same storage places."""
+    L = _lib.lib()
+    h = ctypes.c_void_p(handle)
+    rank, nb = ctypes.c_int32(), ctypes.c_int64()
+    _lib.check(L.tns_shape_counts(h, None, ctypes.byref(rank), None, ctypes.byref(nb), None))
+    pos = np.zeros(max(1, nb.value * rank.value), dtype=np.int64)
+    off = np.zeros(max(1, nb.value), dtype=np.int64)
+    _lib.check(L.tns_shape_blocks(h, pos.ctypes.data_as(_lib.c_i64p), off.ctypes.data_as(_lib.c_i64p)))
+    return pos.tobytes() + off.tobytes()
 
 
 def reweight(psi, possibility):
@@ -205,25 +221,36 @@ class DirectSampler:
         segment of the ket edge ``O0``, in that edge's segment order, read
         in the order (I0, O0) (TAT's transpose of two odd edges changes the
         sign of their odd blocks), negative rounding clipped.  Returns
-        (weights, occupation of each column).  Every sample's hole has the
-        same two edges (the site's physical edge and its conjugate), so one
-        layout serves all items of equal size; the weights of all samples
-        come to the host in one copy."""
+        (weights, occupation of each column).  Items are grouped by their
+        block list (positions and offsets): one layout serves the items of a
+        group; the weights of all samples come to the host in one copy."""
         if sorted(hole.names) != ["I0", "O0"]:
             raise RuntimeError(f"site hole has edges {hole.names}, expected I0 and O0")
         swap = hole.names == ["O0", "I0"]
         kind = hole.model.kind
-        idx0, sgn0, occ = self._layout(hole.tensor(0), kind, swap)
-        rows = np.zeros((len(hole), len(idx0)))
-        for b in np.unique(hole.buf_idx):
-            host = hole.bufs[b].double().cpu().numpy()
-            sel = np.nonzero(hole.buf_idx == b)[0]
-            same = sel[hole.sizes[sel] == hole.sizes[0]]
-            at = np.minimum(hole.offs[same][:, None] + np.maximum(idx0, 0)[None, :], len(host) - 1)
-            rows[same] = host[at] * sgn0
-            for i in sel[hole.sizes[sel] != hole.sizes[0]]:
-                idx, sgn, _ = self._layout(hole.tensor(i), kind, swap)
-                rows[i] = host[np.minimum(hole.offs[i] + np.maximum(idx, 0), len(host) - 1)] * sgn
+        groups = {}
+        for i in range(len(hole)):
+            groups.setdefault(_block_key(int(hole.handles[i])), []).append(i)
+        layouts, occ = {}, None
+        for key, items in groups.items():
+            idx, sgn, o = self._layout(hole.tensor(items[0]), kind, swap)
+            if occ is None:
+                occ = o
+            elif not np.array_equal(occ, o):
+                raise RuntimeError("site holes of one step differ in their physical segments")
+            layouts[key] = (idx, sgn)
+        rows = np.zeros((len(hole), len(occ)))
+        hosts = {int(b): hole.bufs[b].double().cpu().numpy() for b in np.unique(hole.buf_idx)}
+        for key, items in groups.items():
+            idx, sgn = layouts[key]
+            items = np.asarray(items)
+            for b in np.unique(hole.buf_idx[items]):
+                sel = items[hole.buf_idx[items] == b]
+                host = hosts[int(b)]
+                if np.any(idx >= hole.sizes[sel][:, None]):
+                    raise RuntimeError("site hole layout exceeds the item's storage")
+                at = hole.offs[sel][:, None] + np.maximum(idx, 0)[None, :]
+                rows[sel] = np.where(idx[None, :] >= 0, host[at], 0.0) * sgn
         return np.maximum(rows, 0.0), occ
 
     # -- sampling (sampling.py:276-360) -----------------------------------------------------

@@ -213,6 +213,9 @@ class LearnedSampler:
                      logq_r.cpu().numpy().astype(np.float64) - logq_f.cpu().numpy().astype(np.float64))
         u = self.gen.random(B)
         acc = ok & (np.log(np.maximum(u, 1e-300)) < log_a) & (psi_new != 0)
+        # what the step decided on, for callers that check the acceptance rule
+        self.last = {"hops": hops_h, "valid": ok, "proposed": new, "psi_new": psi_new, "log_a": log_a,
+                     "uniforms": u, "accepted": acc}
         self.configs[acc] = new[acc]
         self.psi[acc] = psi_new[acc]
         self.proposals += int(ok.sum())
