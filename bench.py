@@ -35,10 +35,12 @@ def parse():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=4, help="concurrent lockstep lanes (host thread + HIP stream each)")
     ap.add_argument("--workload", default="C4", choices=["C2", "C3", "C4"])
-    ap.add_argument("--cpu-configs", type=int, default=2, help="configs per host process for the CPU baseline")
+    ap.add_argument("--cpu-configs", type=int, default=4, help="configs per host process for the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain", type=int, default=2, help="also time N configs on the plain per-config path")
+    ap.add_argument("--plain", type=int, default=4,
+                    help="also time N configs of the last timed batch on the plain per-config path and compare "
+                         "their psi with the timed engine's")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="contraction GEMM precision of the timed engine")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile per size class")
@@ -171,6 +173,9 @@ def sweep_timing(args, log, eng=None):
     torch.cuda.synchronize()
     de = time.time() - t0
     return {"workload": args.sweep, "chains": len(start), "lanes": args.lanes, "diagonal": args.ess_diagonal,
+            # "keep" skips tetragono's hole(()) refresh of doubly occupied
+            # bonds: a cheaper chain than the reference's at a truncating cut
+            "reference_chain": args.ess_diagonal == "hole",
             "sweep_s": dt, "chain_sweeps_per_s": len(start) / dt, "bonds": len(smp.bonds),
             "proposals_per_s": smp.proposals / (args.sweep_count + 1) / dt,
             "acceptance": smp.accepted / max(1, smp.proposals), "hole_refreshes": smp.refreshed, "observer_s": de,
@@ -187,6 +192,57 @@ def measured_traffic(kernel, args):
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
     return k["traffic_bytes"] if k else None
+
+
+def host_cores():
+    """CPU threads this process may use: the cgroup CPU quota when one is set
+    (a GPU box shares its host), else the affinity mask; and nproc."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            usable = min(usable, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return usable, nproc
+
+
+def kernel_rates(kstats):
+    """Achieved rate of each kernel kind over the timed launches (HIP events
+    on the launch stream): flops for GEMM / QR / SVD against the fp64 matrix
+    peak, bytes for permute / fill / diag against HBM."""
+    out = {}
+    for kind, s in kstats.items():
+        if s["launches"] == 0 or s["ms"] <= 0:
+            continue
+        rate = s["units"] / (s["ms"] / 1e3)
+        if kind in ("permute", "fill", "diag"):
+            out[kind] = {"GB/s": rate / 1e9, "frac_hbm": rate / PEAK["hbm"]}
+        else:
+            out[kind] = {"GFLOP/s": rate / 1e9, "frac_fp64": rate / PEAK["fp64"]}
+    return out
+
+
+def psi_gate(psi_engine, psi_plain, bound=1e-6):
+    """|d log psi| and sign agreement of the timed engine against the plain
+    fp64 path on the same configurations (zeros must agree exactly)."""
+    import numpy as np
+
+    a = np.asarray(psi_engine, dtype=np.float64)
+    b = np.asarray(psi_plain, dtype=np.float64)
+    both = (a != 0) & (b != 0)
+    zeros_agree = bool(np.all((a == 0) == (b == 0)))
+    dl = np.abs(np.log(np.abs(a[both])) - np.log(np.abs(b[both]))) if both.any() else np.zeros(0)
+    sign = float(np.mean(np.sign(a[both]) == np.sign(b[both]))) if both.any() else 1.0
+    worst = float(dl.max()) if dl.size else 0.0
+    return {"configs": int(a.size), "nonzero": int(both.sum()), "max_abs_dlogpsi": worst,
+            "sign_agreement": sign, "zeros_agree": zeros_agree, "bound": bound,
+            "gate_met": bool(worst <= bound and sign == 1.0 and zeros_agree)}
 
 
 def dominant_kernel(kstats):
@@ -305,6 +361,7 @@ def ess_comparison(args, log):
                             log=lambda m: log(f"environment sweep: {m}"), extra=lambda: density(sweep.configs))
     res_sweep["acceptance"] = sweep.accepted / max(1, sweep.proposals)
     res_sweep["diagonal"] = args.ess_diagonal
+    res_sweep["reference_chain"] = args.ess_diagonal == "hole"
     res["sweep"] = res_sweep
     if args.ess_engine_sweep:
         plain = TVSampler(eng, sweep.configs.copy(), seed=1)
@@ -409,22 +466,35 @@ def main():
     roof = dom[1] if dom else None
     if roof:
         roof["traffic"] = measured_traffic(dom[0], args)
+        roof["traffic_source"] = ("committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE profile of this workload "
+                                  "(profiles/c4_traffic.json), not measured in this run"
+                                  if roof["traffic"] is not None else None)
     plain = None
+    gate = None
     if args.plain:
         # the plain path: the drop-in Tensor API, one configuration and one
-        # op at a time (tetragono's own loop on the GPU); first call warms up
+        # op at a time (tetragono's own loop on the GPU), on configurations of
+        # the last timed batch; its psi is the fp64 reference of the gate
         peps.amplitude_plain(lat, configs[0])
         torch.cuda.synchronize()
+        i0 = (args.warmup + args.steps - 1) * B
+        sel = configs[i0:i0 + args.plain]
         t1 = time.time()
-        for c in configs[1:1 + args.plain]:
-            peps.amplitude_plain(lat, c)
+        plain_psi = [peps.amplitude_plain(lat, c)[0] for c in sel]
         torch.cuda.synchronize()
-        plain = args.plain / (time.time() - t1)
-        print(f"[bench] plain per-configuration path {plain:.2f} configs/s", file=sys.stderr, flush=True)
+        plain = len(sel) / (time.time() - t1)
+        gate = psi_gate(psis[-1][:len(sel)], plain_psi)
+        print(f"[bench] plain per-configuration path {plain:.2f} configs/s; gate {gate}", file=sys.stderr,
+              flush=True)
     cpu = None
     if not args.no_cpu:
-        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        usable, nproc = host_cores()
+        procs = args.cpu_procs or usable
         cpu = cpu_baseline(spec, storages, configs, args.cpu_configs, procs)
+        if cpu:
+            cpu["nproc"] = nproc
+            cpu["sample"] += (f"; {procs} processes = the CPU threads this process may use (cgroup quota / "
+                              f"affinity), nproc reports {nproc}")
     nonzero = float(np.mean(psi != 0))
     ess = None
     if args.ess:
@@ -445,8 +515,13 @@ def main():
             direct = direct_timing(args, log)
         except Exception as e:  # noqa: BLE001
             direct = {"error": f"{type(e).__name__}: {e}"}
+    if args.workload == "C4" and args.precision == "fp64":
+        metric = "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec"
+    else:
+        metric = (f"configs/sec {spec.L1}x{spec.L2} {spec.kind} Dc={spec.cut}, contraction GEMMs "
+                  f"{args.precision} (|dlogpsi| vs plain FP64 in parity_gate)")
     line = {
-        "metric": "configs/sec 8x8 D=8 Dc=64 at |dlogpsi|<=1e-6 vs plain FP64; ESS/sec",
+        "metric": metric,
         "value": value,
         "unit": "configs/s",
         "n_gpus": 1,
@@ -465,7 +540,9 @@ def main():
                    "batch": B, "threads": threads, "lanes": args.lanes},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "parity_gate": gate,
         "extra": {"kernel_ms": {k: v["ms"] for k, v in ks.items()},
+                  "kernel_rates": kernel_rates(ks),
                   "kernel_launches": {k: v["launches"] for k, v in ks.items()},
                   "host_plan_s": st["plan_s"], "host_bind_s": st["bind_s"], "svals_wait_s": st["sync_s"],
                   "engine_total_s": st["total_s"],

@@ -243,3 +243,42 @@ def test_svd_not_converged_raises():
         assert L.tns_set_svd_max_sweeps(60) == 0
     u, s, v = C.svd({"i", "j"}, "cu", "cv", "su", "sv", 16)
     assert np.all(np.isfinite(np.asarray(s.storage)))
+
+
+def test_transpose_large_dims_bit_exact():
+    """Tasks far larger than one workgroup chunk, dimensions beyond 2^10 and
+    odd sizes: every permutation equals numpy's transpose bit for bit."""
+    from tnsp_amd import TAT
+
+    dims = [301, 7, 1103]
+    a = TAT.No.D.Tensor(["a", "b", "c"], dims)
+    x = np.random.default_rng(5).standard_normal(int(np.prod(dims)))
+    a.storage = x
+    x = x.reshape(dims)
+    for perm in itertools.permutations(range(3)):
+        names = [["a", "b", "c"][p] for p in perm]
+        assert np.array_equal(storage(a.transpose(names)), np.transpose(x, perm).reshape(-1)), perm
+
+
+@pytest.mark.parametrize("dtype,tol", [("float64", 1e-12), ("float32", 2e-5)])
+@pytest.mark.parametrize("rank_deficient", [False, True])
+@pytest.mark.parametrize("cols", [177, 96, 20])
+def test_qr_tall_cholesky_path(dtype, tol, rank_deficient, cols):
+    """Tall blocks beyond LDS (1200 rows) take the CholeskyQR2 kernel; a block
+    with two equal columns breaks its Cholesky and takes the Householder
+    body instead.  Either way Q R = A, Q^T Q = 1 and R is upper triangular."""
+    from tnsp_amd import TAT
+
+    rng = np.random.default_rng(cols)
+    x = rng.standard_normal((1200, cols))
+    if rank_deficient:
+        x[:, cols // 2] = x[:, 1]
+    mod = TAT.No.D if dtype == "float64" else TAT.No.S
+    a = mod.Tensor(["a", "b"], [1200, cols])
+    a.storage = x.reshape(-1)
+    q, r = a.qr("r", {"b"}, "Q", "R")
+    qm = np.asarray(storage(q.transpose(["a", "Q"])), dtype=np.float64).reshape(1200, cols)
+    rm = np.asarray(storage(r.transpose(["R", "b"])), dtype=np.float64).reshape(cols, cols)
+    assert np.abs(qm @ rm - x).max() < tol * np.abs(x).max() * 10
+    assert np.abs(qm.T @ qm - np.eye(cols)).max() < tol * 10
+    assert np.array_equal(np.tril(rm, -1), np.zeros_like(rm))

@@ -26,21 +26,83 @@ __device__ inline T neg_if(T v, int neg) {
 }
 
 // ---------------------------------------------------------------- permute
-// Each thread handles PERM_CHUNK / WG elements, WG apart: the offsets of all
-// of them are decoded first, then all loads are issued before the stores, so
-// a thread keeps that many gathers in flight instead of one.
+// floor(x / d) for 0 <= x < 2^31 from d's round-up magic (perm_magic)
+__device__ inline uint32_t magic_div(uint32_t x, uint32_t mul, int sh) {
+    return uint32_t((uint64_t(x) * mul) >> sh);
+}
+
+// source and destination offsets of element `local` of a task, decoded
+// row-major over dims (destination order, last dimension fastest)
+__device__ inline void perm_decode(uint32_t local, int rank, const int32_t* dims, const int32_t* ss,
+                                   const int32_t* ds, const uint32_t* mul, const int32_t* sh, int64_t& a,
+                                   int64_t& c) {
+    a = 0;
+    c = 0;
+#pragma unroll
+    for (int r = DEV_MAXR - 1; r >= 0; r--) {
+        if (r < rank) {
+            const uint32_t q = magic_div(local, mul[r], sh[r]);
+            const uint32_t i = local - q * uint32_t(dims[r]);
+            a += int64_t(i) * ss[r];
+            c += int64_t(i) * ds[r];
+            local = q;
+        }
+    }
+}
+
+// Each workgroup copies PERM_CHUNK consecutive elements of the flat element
+// space of all tasks; each thread handles PERM_CHUNK / WG of them, WG apart,
+// all loads issued before the stores.  A chunk inside one task (the common
+// case: most elements belong to tasks larger than a chunk) reads that task's
+// descriptor once for the whole workgroup; a chunk spanning several tasks
+// locates each element's task through the prefix sums.
 template<typename TS, typename T>
 __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__ tasks, const int64_t* __restrict__ prefix,
                                                      const int32_t* __restrict__ wg_first) {
     constexpr int PER = PERM_CHUNK / WG;
     const int64_t base = int64_t(blockIdx.x) * PERM_CHUNK;
+    const int t0 = wg_first[blockIdx.x];
     const int t_end = wg_first[gridDim.x];
-    int t = wg_first[blockIdx.x];
+    const int64_t p0 = prefix[t0], p1 = prefix[t0 + 1];
+    if (base + PERM_CHUNK <= p1 || t0 + 1 == t_end) {
+        const PermDev& tk = tasks[t0];
+        const int rank = tk.rank;
+        int32_t dims[DEV_MAXR], ss[DEV_MAXR], ds[DEV_MAXR], sh[DEV_MAXR];
+        uint32_t mul[DEV_MAXR];
+#pragma unroll
+        for (int r = 0; r < DEV_MAXR; r++) {
+            dims[r] = tk.dims[r];
+            ss[r] = tk.ss[r];
+            ds[r] = tk.ds[r];
+            mul[r] = tk.mul[r];
+            sh[r] = tk.sh[r];
+        }
+        const TS* src = static_cast<const TS*>(tk.src);
+        T* dst = static_cast<T*>(tk.dst);
+        const bool neg = tk.neg != 0;
+        const int64_t left = p1 - base;  // elements of task t0 from base on
+        const uint32_t off = uint32_t(base - p0);
+        int64_t so[PER], dof[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int e = threadIdx.x + j * WG;
+            so[j] = dof[j] = 0;
+            if (e < left) perm_decode(off + uint32_t(e), rank, dims, ss, ds, mul, sh, so[j], dof[j]);
+        }
+        T v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) v[j] = (threadIdx.x + j * WG < left) ? T(src[so[j]]) : T(0);
+#pragma unroll
+        for (int j = 0; j < PER; j++)
+            if (threadIdx.x + j * WG < left) dst[dof[j]] = neg_if(v[j], neg ? 1 : 0);
+        return;
+    }
+    int t = t0;
     const TS* src[PER];
     T* dst[PER];
     int64_t so[PER], dof[PER];
     bool neg[PER], ok[PER];
-    int64_t end = prefix[t + 1];
+    int64_t end = p1;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
         const int64_t e = base + threadIdx.x + int64_t(j) * WG;
@@ -56,17 +118,7 @@ __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__
         neg[j] = false;
         if (ok[j]) {
             const PermDev& tk = tasks[t];
-            uint32_t local = uint32_t(e - prefix[t]);
-            int64_t a = 0, c = 0;
-            for (int r = tk.rank - 1; r >= 0; r--) {
-                const uint32_t d = uint32_t(tk.dims[r]);
-                const uint32_t i = local % d;
-                local /= d;
-                a += int64_t(i) * tk.ss[r];
-                c += int64_t(i) * tk.ds[r];
-            }
-            so[j] = a;
-            dof[j] = c;
+            perm_decode(uint32_t(e - prefix[t]), tk.rank, tk.dims, tk.ss, tk.ds, tk.mul, tk.sh, so[j], dof[j]);
             src[j] = static_cast<const TS*>(tk.src);
             dst[j] = static_cast<T*>(tk.dst);
             neg[j] = tk.neg != 0;
@@ -153,12 +205,16 @@ __device__ inline void bf16_split(float x, s4* terms, int j) {
 // Mfma<T>::row.  Variants: 32x32 (2x2 waves, 1x1 fragments), 64x64 (2x2
 // waves, 2x2 fragments: each operand fragment feeds two MFMAs), 16x16 (one
 // wave, for sectors of at most 16 rows and columns).
+// The K loop is software-pipelined: the global loads of k-block kb + 1 are
+// issued into registers before the MFMAs of block kb run from one LDS
+// buffer, then land in the other buffer; one barrier per k-block.
 template<typename T, int WR, int WC, int FM, int FN>
 __global__ __launch_bounds__(64 * WR * WC) void gemm_kernel(const GemmDev* __restrict__ tasks,
                                                            const TileDev* __restrict__ tiles) {
     constexpr int NT = 64 * WR * WC, TM = 16 * WR * FM, TN = 16 * WC * FN;
-    __shared__ T As[TM][KB + 1];
-    __shared__ T Bs[KB][TN + 1];
+    constexpr int LA = (TM * KB + NT - 1) / NT, LB = (KB * TN + NT - 1) / NT;
+    __shared__ T As[2][TM][KB + 1];
+    __shared__ T Bs[2][KB][TN + 1];
     const TileDev tile = tiles[blockIdx.x];
     const GemmDev g = tasks[tile.task];
     const T* A = static_cast<const T*>(g.a);
@@ -171,50 +227,88 @@ __global__ __launch_bounds__(64 * WR * WC) void gemm_kernel(const GemmDev* __res
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::zero();
-    for (int k0 = 0; k0 < g.k; k0 += KB) {
-        for (int e = tid; e < TM * KB; e += NT) {
-            // A tile: rows row0.., cols k0..; index so that consecutive threads
-            // walk the contiguous direction of the stored operand
+    // element u of this thread's share of a k-block: A tile (i, p) and B
+    // tile (p, j), indexed so that consecutive threads walk the contiguous
+    // direction of the stored operand
+    auto a_pos = [&](int e, int& i, int& p) {
+        if (g.ta) {
+            i = e % TM;
+            p = e / TM;
+        } else {
+            p = e % KB;
+            i = e / KB;
+        }
+    };
+    auto b_pos = [&](int e, int& p, int& j) {
+        if (g.tb) {
+            p = e % KB;
+            j = e / KB;
+        } else {
+            j = e % TN;
+            p = e / TN;
+        }
+    };
+    T ra[LA], rb[LB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < LA; u++) {
+            const int e = tid + u * NT;
             int i, p;
-            if (g.ta) {
-                i = e % TM;
-                p = e / TM;
-            } else {
-                p = e % KB;
-                i = e / KB;
-            }
+            a_pos(e, i, p);
             const int gi = row0 + i, gp = k0 + p;
-            T v = T(0);
-            if (gi < g.m && gp < g.k) v = g.ta ? A[int64_t(gp) * g.lda + gi] : A[int64_t(gi) * g.lda + gp];
-            As[i][p] = v;
+            ra[u] = (e < TM * KB && gi < g.m && gp < g.k)
+                        ? (g.ta ? A[int64_t(gp) * g.lda + gi] : A[int64_t(gi) * g.lda + gp])
+                        : T(0);
         }
-        for (int e = tid; e < TN * KB; e += NT) {
+#pragma unroll
+        for (int u = 0; u < LB; u++) {
+            const int e = tid + u * NT;
             int p, j;
-            if (g.tb) {
-                p = e % KB;
-                j = e / KB;
-            } else {
-                j = e % TN;
-                p = e / TN;
-            }
+            b_pos(e, p, j);
             const int gp = k0 + p, gj = col0 + j;
-            T v = T(0);
-            if (gp < g.k && gj < g.n) v = g.tb ? B[int64_t(gj) * g.ldb + gp] : B[int64_t(gp) * g.ldb + gj];
-            Bs[p][j] = v;
+            rb[u] = (e < KB * TN && gp < g.k && gj < g.n)
+                        ? (g.tb ? B[int64_t(gj) * g.ldb + gp] : B[int64_t(gp) * g.ldb + gj])
+                        : T(0);
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < LA; u++) {
+            const int e = tid + u * NT;
+            int i, p;
+            a_pos(e, i, p);
+            if (e < TM * KB) As[buf][i][p] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; u++) {
+            const int e = tid + u * NT;
+            int p, j;
+            b_pos(e, p, j);
+            if (e < KB * TN) Bs[buf][p][j] = rb[u];
+        }
+    };
+    const int nk = (g.k + KB - 1) / KB;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kb = 0; kb < nk; kb++) {
+        const int cur = kb & 1;
+        if (kb + 1 < nk) load((kb + 1) * KB);
 #pragma unroll
         for (int kk = 0; kk < KB; kk += 4) {
             T a[FM], b[FN];
 #pragma unroll
-            for (int i = 0; i < FM; i++) a[i] = As[wr + 16 * i + lane % 16][kk + lane / 16];
+            for (int i = 0; i < FM; i++) a[i] = As[cur][wr + 16 * i + lane % 16][kk + lane / 16];
 #pragma unroll
-            for (int j = 0; j < FN; j++) b[j] = Bs[kk + lane / 16][wc + 16 * j + lane % 16];
+            for (int j = 0; j < FN; j++) b[j] = Bs[cur][kk + lane / 16][wc + 16 * j + lane % 16];
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
                 for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::step(a[i], b[j], acc[i][j]);
         }
+        // buffer cur ^ 1 was last read in iteration kb - 1, before that
+        // iteration's barrier
+        if (kb + 1 < nk) store(cur ^ 1);
         __syncthreads();
     }
     T* C = static_cast<T*>(g.c);
@@ -303,6 +397,57 @@ __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict
 }
 
 // --------------------------------------------------------- reductions
+constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111, DPP_ROW_ROR8 = 0x128;
+
+// a double moved across lanes by a DPP row shift (two 32-bit halves)
+template<int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
+}
+
+// v of lane l plus v of lane l ^ 16 (SWAP = 16) or l ^ 32 (SWAP = 32), by the
+// gfx950 row / half swaps: permlane16_swap(x, x) leaves each lane one of
+// {x_l, x_(l^16)} in each of its two results (even rows keep their own value
+// in the first and receive the partner's in the second, odd rows the other
+// way round), so the two results add up to the pair sum in every lane; the
+// same holds for permlane32_swap with halves.  VALU only, no LDS.
+template<int SWAP>
+__device__ inline double swap_sum(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(u), hi = unsigned(u >> 32);
+    unsigned lo0, lo1, hi0, hi1;
+    if constexpr (SWAP == 16) {
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        lo0 = l[0];
+        lo1 = l[1];
+        hi0 = h[0];
+        hi1 = h[1];
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+        lo0 = l[0];
+        lo1 = l[1];
+        hi0 = h[0];
+        hi1 = h[1];
+    }
+    const double a = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi0) << 32) | lo0);
+    const double b = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi1) << 32) | lo1);
+    return a + b;
+}
+
+// butterfly sum over the 8 lanes {l ^ 8k}: l ^ 8 by a DPP rotate by 8 within
+// the 16-lane row, l ^ 16 and l ^ 32 by the row and half swaps; every lane
+// ends with the same value (the additions commute)
+__device__ inline double sum8(double v) {
+    v += dpp_f64<DPP_ROW_ROR8>(v);
+    v = swap_sum<16>(v);
+    return swap_sum<32>(v);
+}
+
 __device__ inline double wave_sum(double v) {
     for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
     return v;
@@ -727,6 +872,296 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
     }
 }
 
+// ----------------------------------------------------------------- cholqr
+// CholeskyQR2 for tall QR sectors beyond LDS (M = Q R with m >= n, n <=
+// 16 * QC_T): one workgroup of QR_WG threads per sector, every product on the
+// fp64 MFMA (16x16x4):
+//   G = M^T M                  (upper 16x16 tiles, spread over the waves,
+//                               accumulated in registers)
+//   G = R1^T R1                (tiled right-looking Cholesky, tiles kept in
+//                               registers; each diagonal tile factored and
+//                               inverted in LDS by one wave)
+//   Q1 = M R1^-1               (block forward substitution over the column
+//                               tiles, 16 rows per wave at a time)
+// then the same on Q1 (R2, Q = Q1 R2^-1) and R = R2 R1.  M is read twice, Q1
+// written once and read twice: the traffic of about six passes over the
+// block instead of two per Householder panel.  Orthogonality and residual are
+// at rounding level while cond(M) is far below 1e8; a Cholesky pivot below
+// 1e-10 of its column's squared norm (or not positive) sends the whole sector
+// to the Householder kernel body instead, so rank-deficient blocks still get
+// LAPACK's answer.  Scratch: Q1 (m n) and the R1 / R2 tiles and the inverses
+// of the diagonal tiles (QC_EXTRA doubles).
+constexpr int QC_NW = QR_WG / WAVE;
+constexpr int QC_NTL = QC_T * (QC_T + 1) / 2;
+constexpr int QC_SLOTS = (QC_NTL + QC_NW - 1) / QC_NW;
+
+// upper tile t (row-major over I <= J) -> I, J
+__host__ __device__ constexpr int qc_tile_i(int t) {
+    int i = 0;
+    while (t >= QC_T - i) {
+        t -= QC_T - i;
+        i++;
+    }
+    return i;
+}
+__host__ __device__ constexpr int qc_tile_j(int t) {
+    int i = 0;
+    while (t >= QC_T - i) {
+        t -= QC_T - i;
+        i++;
+    }
+    return i + t;
+}
+__host__ __device__ constexpr int qc_tile(int i, int j) { return i * QC_T - i * (i - 1) / 2 + (j - i); }
+
+__device__ inline d4 qc_mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// acc[t / QC_NW] of wave t % QC_NW = tile t of X^T X (X: m x n row-major)
+template<typename S>
+__device__ inline void qc_gram(const S* __restrict__ X, int m, int n, int nt, d4* acc) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int s = 0; s < QC_SLOTS; s++) acc[s] = d4{0.0, 0.0, 0.0, 0.0};
+    const int c = lane % 16;
+    for (int k0 = 0; k0 < m; k0 += 4) {
+        const int r = k0 + lane / 16;
+        double b[QC_T];
+#pragma unroll
+        for (int J = 0; J < QC_T; J++) {
+            const int col = 16 * J + c;
+            b[J] = (J < nt && r < m && col < n) ? double(X[int64_t(r) * n + col]) : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < QC_NTL; t++) {
+            if (t % QC_NW == wave && qc_tile_j(t) < nt)
+                acc[t / QC_NW] = qc_mfma(b[qc_tile_i(t)], b[qc_tile_j(t)], acc[t / QC_NW]);
+        }
+    }
+}
+
+// In-place tiled Cholesky of the register tiles (acc: G -> R, upper); R
+// tiles also go row-major to Rg (tile (I, J) at (I QC_T + J) 256), the
+// inverses of the diagonal tiles to RIg (tile J at J 256).  Sets *fail when a
+// pivot falls below 1e-10 of its column's squared norm.
+__device__ inline void qc_cholesky(d4* acc, int n, int nt, double* __restrict__ Rg, double* __restrict__ RIg,
+                                   double (*D)[17], double (*E)[17], double (*Rrow)[16][16], double* gdiag,
+                                   int* fail) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    // squared column norms (diagonal of G) for the breakdown test
+#pragma unroll
+    for (int K = 0; K < QC_T; K++) {
+        const int t = qc_tile(K, K);
+        if (K < nt && t % QC_NW == wave) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (lane / 16 + 4 * i == lane % 16) gdiag[16 * K + lane % 16] = acc[t / QC_NW][i];
+        }
+    }
+#pragma unroll
+    for (int K = 0; K < QC_T; K++) {
+        if (K >= nt) break;
+        const int tkk = qc_tile(K, K);
+        if (tkk % QC_NW == wave) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) D[lane / 16 + 4 * i][lane % 16] = acc[tkk / QC_NW][i];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // unblocked Cholesky of the 16x16 tile (upper R with R^T R = D)
+            for (int c = 0; c < 16; c++) {
+                if (lane == 0) {
+                    // columns past n (zero padding of the last tile) get a
+                    // unit pivot: their rows and columns of G are zero
+                    const double p = D[c][c];
+                    const double g = gdiag[16 * K + c];
+                    const bool pad = 16 * K + c >= n;
+                    if (!pad && (!(p > 1e-10 * g) || !(g > 0.0))) *fail = 1;
+                    D[c][c] = (pad || !(p > 0.0)) ? 1.0 : sqrt(p);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (lane > c && lane < 16) D[c][lane] /= D[c][c];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int e = lane; e < 256; e += WAVE) {
+                    const int i = e / 16, j = e % 16;
+                    if (i > c && j >= i) D[i][j] -= D[c][i] * D[c][j];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+            for (int e = lane; e < 256; e += WAVE) {
+                const int i = e / 16, j = e % 16;
+                if (i > j) D[i][j] = 0.0;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // E = D^-1 (upper), column j by lane j, back substitution
+            if (lane < 16) {
+                const int j = lane;
+                for (int i = 15; i >= 0; i--) {
+                    double v;
+                    if (i > j) {
+                        v = 0.0;
+                    } else if (i == j) {
+                        v = 1.0 / D[j][j];
+                    } else {
+                        double sacc = 0.0;
+                        for (int l = i + 1; l <= j; l++) sacc += D[i][l] * E[l][j];
+                        v = -sacc / D[i][i];
+                    }
+                    E[i][j] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // row K: R_KK from D, R_KJ = R_KK^-T G_KJ (A = E^T, B = the tile)
+#pragma unroll
+        for (int J = 0; J < QC_T; J++) {
+            if (J < K) continue;
+            const int t = qc_tile(K, J);
+            if (J >= nt || t % QC_NW != wave) continue;
+            d4 r;
+            if (J == K) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) r[i] = D[lane / 16 + 4 * i][lane % 16];
+#pragma unroll
+                for (int i = 0; i < 4; i++) RIg[K * 256 + 64 * i + lane] = E[lane / 16 + 4 * i][lane % 16];
+            } else {
+                r = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < 4; s++) r = qc_mfma(E[4 * s + lane / 16][lane % 16], acc[t / QC_NW][s], r);
+            }
+            acc[t / QC_NW] = r;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                Rrow[J][lane / 16 + 4 * i][lane % 16] = r[i];
+                Rg[(K * QC_T + J) * 256 + 64 * i + lane] = r[i];
+            }
+        }
+        __syncthreads();
+        // trailing tiles (I, J), K < I <= J: G_IJ -= R_KI^T R_KJ
+#pragma unroll
+        for (int t = 0; t < QC_NTL; t++) {
+            const int I = qc_tile_i(t), J = qc_tile_j(t);
+            if (I <= K || J >= nt || t % QC_NW != wave) continue;
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                acc[t / QC_NW] = qc_mfma(-Rrow[I][4 * s + lane / 16][lane % 16], Rrow[J][4 * s + lane / 16][lane % 16],
+                                         acc[t / QC_NW]);
+        }
+    }
+    __syncthreads();
+}
+
+// Q = X R^-1 by block forward substitution over the column tiles: per 16-row
+// block, tile J of Q^T = R_JJ^-T (X_J^T - sum_{I<J} R_IJ^T Q_I^T), the tiles of
+// Q^T in MFMA accumulator layout (which is the B-operand layout of the next
+// products)
+template<typename S, typename TO>
+__device__ inline void qc_solve(const S* __restrict__ X, int m, int n, int nt, const double* __restrict__ Rg,
+                                const double* __restrict__ RIg, TO* __restrict__ Q) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    const int nrb = (m + 15) / 16;
+    for (int rb = wave; rb < nrb; rb += QC_NW) {
+        const int r = 16 * rb + lane % 16;
+        d4 q[QC_T];
+#pragma unroll
+        for (int J = 0; J < QC_T; J++) {
+            if (J >= nt) break;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int col = 16 * J + lane / 16 + 4 * i;
+                q[J][i] = (r < m && col < n) ? double(X[int64_t(r) * n + col]) : 0.0;
+            }
+#pragma unroll
+            for (int I = 0; I < QC_T; I++) {
+                if (I >= J) break;
+                const double* rt = Rg + (I * QC_T + J) * 256;
+#pragma unroll
+                for (int s = 0; s < 4; s++) q[J] = qc_mfma(-rt[64 * s + lane], q[I][s], q[J]);
+            }
+            d4 o = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < 4; s++) o = qc_mfma(RIg[J * 256 + 64 * s + lane], q[J][s], o);
+            q[J] = o;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int col = 16 * J + lane / 16 + 4 * i;
+                if (r < m && col < n) Q[int64_t(r) * n + col] = TO(o[i]);
+            }
+        }
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(QR_WG) void qr_kernel_chol(const QrDev* __restrict__ tasks) {
+    __shared__ double D[16][17];
+    __shared__ double E[16][17];
+    __shared__ double Rrow[QC_T][16][16];
+    __shared__ double gdiag[16 * QC_T];
+    __shared__ double red[QR_WG / WAVE];
+    __shared__ double bc[2];
+    __shared__ int fail;
+    const QrDev tk = tasks[blockIdx.x];
+    const int m = tk.m, n = tk.n, nt = (n + 15) / 16;
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (tk.lq || m < n || nt > QC_T) {
+        // the host routes only fitting blocks here
+        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
+        return;
+    }
+    double* Q1 = tk.work;
+    double* R1 = Q1 + int64_t(m) * n;
+    double* RI = R1 + QC_T * QC_T * 256;
+    double* R2 = RI + QC_T * 256;
+    const T* M = static_cast<const T*>(tk.in);
+    if (threadIdx.x == 0) fail = 0;
+    __syncthreads();
+    d4 acc[QC_SLOTS];
+    qc_gram(M, m, n, nt, acc);
+    qc_cholesky(acc, n, nt, R1, RI, D, E, Rrow, gdiag, &fail);
+    if (fail) {
+        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
+        return;
+    }
+    qc_solve(M, m, n, nt, R1, RI, Q1);
+    __syncthreads();
+    qc_gram(Q1, m, n, nt, acc);
+    qc_cholesky(acc, n, nt, R2, RI, D, E, Rrow, gdiag, &fail);
+    if (fail) {
+        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
+        return;
+    }
+    T* Qo = static_cast<T*>(tk.f1);
+    T* Ro = static_cast<T*>(tk.f2);
+    qc_solve(Q1, m, n, nt, R2, RI, Qo);
+    // R = R2 R1 (upper tiles), lower tiles zero; R is n x n row-major
+#pragma unroll
+    for (int t = 0; t < QC_NTL; t++) {
+        const int I = qc_tile_i(t), J = qc_tile_j(t);
+        if (J >= nt || t % QC_NW != wave) continue;
+        d4 r = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int L = 0; L < QC_T; L++) {
+            if (L < I || L > J) continue;
+            const double* a = R2 + (I * QC_T + L) * 256;
+            const double* b = R1 + (L * QC_T + J) * 256;
+#pragma unroll
+            for (int s = 0; s < 4; s++) r = qc_mfma(a[(lane % 16) * 16 + 4 * s + lane / 16], b[64 * s + lane], r);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 16 * I + lane / 16 + 4 * i, col = 16 * J + lane % 16;
+            if (row < n && col < n) Ro[int64_t(row) * n + col] = T(r[i]);
+        }
+    }
+    for (int e = threadIdx.x; e < n * n; e += QR_WG) {
+        const int row = e / n, col = e % n;
+        if (row / 16 > col / 16) Ro[e] = T(0);
+    }
+}
+
 // -------------------------------------------------------------------- svd
 // Ranks the singular values (descending, ties by column) and writes S, U
 // and V^T of the block from X (normalised columns), V and sig.
@@ -950,34 +1385,15 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
 // V = 0 (their products U S V^T are below that level).  The engine only
 // consumes U S and V^T of kept values, so no orthonormal completion is done.
 
-constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111, DPP_ROW_ROR8 = 0x128;
-
-// a double moved across lanes by a DPP row shift (two 32-bit halves)
-template<int CTRL>
-__device__ inline double dpp_f64(double v) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xF, 0xF, true);
-    return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
-}
-
-// butterfly sum over the 8 lanes {l ^ 8k}: the first level by a DPP rotate
-// by 8 within the 16-lane row (lane l ^ 8), the others through LDS shuffles;
-// every lane ends with the same value (the additions commute)
-__device__ inline double sum8(double v) {
-    v += dpp_f64<DPP_ROW_ROR8>(v);
-    v += __shfl_xor(v, 16, WAVE);
-    v += __shfl_xor(v, 32, WAVE);
-    return v;
-}
-
 template<typename T, int RPT>
 __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restrict__ tasks) {
     constexpr int SVR_NT = 32 * RPT, SVR_SLOTS = 4 * RPT, NWV = SVR_NT / WAVE;
-    // wave-boundary columns: 2 parities x (top, bottom) x NWV x 8 x RPT
-    // doubles; reused for the recovery tiles
+    // exchange buffers per round parity: (top, bottom) x NWV x 8 x RPT doubles
+    // at the wave edges plus slot 0's and slot h-1's tops (2 x 8 x RPT);
+    // reused for the recovery tiles
+    constexpr int SVR_XCH = (2 * NWV + 2) * 8 * RPT;
     extern __shared__ double xbuf[];
-    __shared__ int ibuf[2][2][NWV];
+    __shared__ int ibuf[2][3][NWV];
     __shared__ double smax;
     const SvdDev tk = tasks[blockIdx.x];
     const int m = tk.m, n = tk.n;
@@ -1040,17 +1456,24 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
             }
             if (h < 2) continue;
             // ring step: top[p] <- top[p-1] (top[1] <- bot[0], top[0] stays),
-            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1]).  Slots p -+ 1 of the
-            // same wave sit in the neighbouring lane of the same 16-lane row
-            // (DPP row shift); slots across a wave boundary go through LDS,
-            // double-buffered by round parity so one barrier per round
-            // suffices.
+            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1]).  Every lane shifts
+            // both columns by a DPP row shift (slot p -+ 1 of the same wave
+            // is the neighbouring lane of the same 16-lane row); the lanes
+            // whose new column is not that neighbour's (pl = 0 and 7 at the
+            // wave edges, slot 0's fixed top, slot 1's top from slot 0's
+            // bottom, slot h-1's bottom from its own top) were written to LDS
+            // before the barrier and read back in one lane-masked pass each.
+            // The buffers are double-buffered by round parity so one barrier
+            // per round suffices.
             {
                 const int par = round & 1;
-                double* btop = xbuf + par * (2 * NWV * 8 * RPT);
-                double* bbot = btop + NWV * 8 * RPT;
+                double* btop = xbuf + par * SVR_XCH;  // top of slot 8w+7, per wave
+                double* bbot = btop + NWV * 8 * RPT;  // bottom of slot 8w, per wave
+                double* bkeep0 = bbot + NWV * 8 * RPT;  // top of slot 0
+                double* bkeepl = bkeep0 + 8 * RPT;      // top of slot h-1
                 int* itop = ibuf[par][0];
                 int* ibot = ibuf[par][1];
+                const int wl = (h - 1) / 8;  // wave holding slot h-1
                 if (pl == 7) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) btop[(w * 8 + gl) * RPT + i] = xa[i];
@@ -1061,24 +1484,46 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
                     for (int i = 0; i < RPT; i++) bbot[(w * 8 + gl) * RPT + i] = xb[i];
                     if (gl == 0) ibot[w] = cb;
                 }
+                if (p == 0) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) bkeep0[gl * RPT + i] = xa[i];
+                    if (gl == 0) ibuf[par][2][0] = ca;
+                }
+                if (p == h - 1) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) bkeepl[gl * RPT + i] = xa[i];
+                    if (gl == 0) ibuf[par][2][1] = ca;
+                }
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < RPT; i++) {
-                    double t = dpp_f64<DPP_ROW_SHR1>(p == 0 ? xb[i] : xa[i]);
-                    double b = dpp_f64<DPP_ROW_SHL1>(xb[i]);
-                    if (pl == 0) t = w > 0 ? btop[((w - 1) * 8 + gl) * RPT + i] : xa[i];
-                    if (pl == 7) b = w + 1 < NWV ? bbot[((w + 1) * 8 + gl) * RPT + i] : 0.0;
-                    const double nb = p == h - 1 ? xa[i] : b;
-                    if (p != 0) xa[i] = t;
-                    xb[i] = nb;
+                    xa[i] = dpp_f64<DPP_ROW_SHR1>(xa[i]);
+                    xb[i] = dpp_f64<DPP_ROW_SHL1>(xb[i]);
                 }
-                int ti = __builtin_amdgcn_update_dpp(0, p == 0 ? cb : ca, DPP_ROW_SHR1, 0xF, 0xF, true);
-                int bi = __builtin_amdgcn_update_dpp(0, cb, DPP_ROW_SHL1, 0xF, 0xF, true);
-                if (pl == 0) ti = w > 0 ? itop[w - 1] : ca;
-                if (pl == 7) bi = w + 1 < NWV ? ibot[w + 1] : 0;
-                const int ncb = p == h - 1 ? ca : bi;
-                if (p != 0) ca = ti;
-                cb = ncb;
+                ca = __builtin_amdgcn_update_dpp(0, ca, DPP_ROW_SHR1, 0xF, 0xF, true);
+                cb = __builtin_amdgcn_update_dpp(0, cb, DPP_ROW_SHL1, 0xF, 0xF, true);
+                if (pl == 0) {
+                    const double* src = w > 0 ? btop + ((w - 1) * 8 + gl) * RPT : bkeep0 + gl * RPT;
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) xa[i] = src[i];
+                    ca = w > 0 ? itop[w - 1] : ibuf[par][2][0];
+                }
+                if (w == 0 && pl == 1) {
+                    // slot 1's top is slot 0's bottom
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) xa[i] = bbot[gl * RPT + i];
+                    ca = ibot[0];
+                }
+                if (pl == 7 && w + 1 < NWV) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) xb[i] = bbot[((w + 1) * 8 + gl) * RPT + i];
+                    cb = ibot[w + 1];
+                }
+                if (w == wl && p == h - 1) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) xb[i] = bkeepl[gl * RPT + i];
+                    cb = ibuf[par][2][1];
+                }
             }
         }
         if (!__syncthreads_or(rot)) {
@@ -1273,7 +1718,12 @@ void allow_lds(K kernel, int64_t bytes) {
 
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (variant == 1) {
+    if (variant == 2) {
+        if (dtype == 0)
+            hipLaunchKernelGGL(qr_kernel_chol<double>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
+        else
+            hipLaunchKernelGGL(qr_kernel_chol<float>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
+    } else if (variant == 1) {
         if (dtype == 0) {
             allow_lds(qr_kernel_blocked<double>, QB_LDS);
             hipLaunchKernelGGL(qr_kernel_blocked<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
@@ -1303,9 +1753,9 @@ void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, 
 namespace {
 template<typename T, int RPT>
 void launch_svd_reg(const SvdDev* tasks, int64_t num, hipStream_t stream) {
-    // wave-boundary buffer (2 x 2 x waves x 8 x RPT), at least the 32 KiB of
-    // the recovery tiles
-    const size_t lds = std::max<size_t>(size_t(2 * 2 * (RPT / 2) * 8 * RPT), 2 * 32 * 64) * sizeof(double);
+    // exchange buffers (2 parities x (2 x waves + 2) x 8 x RPT), at least the
+    // 32 KiB of the recovery tiles
+    const size_t lds = std::max<size_t>(size_t(2 * (2 * (RPT / 2) + 2) * 8 * RPT), 2 * 32 * 64) * sizeof(double);
     allow_lds(svd_kernel_reg<T, RPT>, int64_t(lds));
     hipLaunchKernelGGL((svd_kernel_reg<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
 }

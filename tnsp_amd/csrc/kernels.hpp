@@ -10,7 +10,9 @@ namespace tns {
 constexpr int DEV_MAXR = 8;
 
 // One strided block copy: dst[i] = (neg ? -1 : 1) * src[i] over a rank-r box,
-// i enumerated row-major over dims (destination order).
+// i enumerated row-major over dims (destination order).  mul / sh: the
+// round-up division magic of each dimension (perm_magic), so that the kernel
+// decodes a flat index with multiply-shifts instead of integer divisions.
 struct PermDev {
     const void* src;
     void* dst;
@@ -19,7 +21,19 @@ struct PermDev {
     int32_t dims[DEV_MAXR];
     int32_t ss[DEV_MAXR];
     int32_t ds[DEV_MAXR];
+    uint32_t mul[DEV_MAXR];
+    int32_t sh[DEV_MAXR];
 };
+
+// Granlund-Montgomery round-up magic for 31-bit dividends: with
+// l = ceil(log2 d), mul = floor(2^(31+l) / d) + 1 (< 2^32 for every d >= 1)
+// and sh = 31 + l, floor(x / d) = (x * mul) >> sh for all 0 <= x < 2^31.
+inline void perm_magic(uint32_t d, uint32_t* mul, int32_t* sh) {
+    int l = 0;
+    while ((uint64_t(1) << l) < d) l++;
+    *mul = uint32_t((uint64_t(1) << (31 + l)) / d + 1);
+    *sh = 31 + l;
+}
 
 struct GemmDev {
     const void* a;
@@ -91,17 +105,27 @@ constexpr int SVD_WG = 256;
 constexpr int64_t LDS_CLASS[3] = {48 * 1024, 76 * 1024, 152 * 1024};
 constexpr int64_t QR_LDS_MAX = LDS_CLASS[2];
 constexpr int64_t SVD_LDS_MAX = LDS_CLASS[2];
-inline int64_t qr_doubles(int64_t m, int64_t n) {
+// CholeskyQR2 kernel (variant 2 of launch_qr): QR (not LQ) blocks with
+// m >= n, at most 16 * QC_T columns, beyond the LDS-resident kernel; its
+// scratch holds Q1 (m n) plus the R1 / R2 tiles and the inverses of the
+// diagonal tiles
+constexpr int QC_T = 12;
+constexpr int64_t QC_EXTRA = int64_t(2 * QC_T * QC_T + QC_T) * 256;
+inline bool qr_uses_chol(int64_t m, int64_t n, int lq) {
+    return !lq && m >= n && n <= 16 * QC_T && (m * n + (m < n ? m : n)) * 8 > LDS_CLASS[2];
+}
+inline int64_t qr_doubles(int64_t m, int64_t n, int lq) {
     const int64_t k = m < n ? m : n;
-    return m * n + k;
+    return m * n + (qr_uses_chol(m, n, lq) ? QC_EXTRA : k);
 }
 inline int64_t svd_doubles(int64_t m, int64_t n) {
     const int64_t mx = m > n ? m : n, kx = m < n ? m : n;
     return mx * kx + kx * kx + kx;
 }
 // variant 1: blocked Householder with the panel in LDS (global scratch
-// blocks with at most QB_MAX_ROWS rows); otherwise LDS (lds_bytes > 0) or
-// unblocked global scratch
+// blocks with at most QB_MAX_ROWS rows); variant 2: CholeskyQR2
+// (qr_uses_chol blocks); otherwise LDS (lds_bytes > 0) or unblocked global
+// scratch
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream);
 constexpr int64_t QB_LDS = 156 * 1024;
 // rows for which the blocked kernel's panel keeps at least 4 columns

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstring>
 #include <stdexcept>
+#include <type_traits>
 
 namespace tns {
 
@@ -121,10 +122,12 @@ PermDev bind_perm(const PermTask& t, void* const* bufs, int64_t es_src, int64_t 
     d.n = t.n;
     d.rank = t.rank;
     d.neg = t.neg;
+    if (t.n >= (int64_t(1) << 31)) throw Error("permute task too large");
     for (int i = 0; i < MAXR; i++) {
         d.dims[i] = t.dims[i];
         d.ss[i] = t.ss[i];
         d.ds[i] = t.ds[i];
+        perm_magic(uint32_t(t.dims[i] > 0 ? t.dims[i] : 1), &d.mul[i], &d.sh[i]);
     }
     return d;
 }
@@ -307,7 +310,7 @@ void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, dou
         q.m = t.m;
         q.n = t.n;
         q.lq = t.lq;
-        cur += qr_doubles(t.m, t.n);
+        cur += qr_doubles(t.m, t.n, t.lq);
         const double M = double(std::max(t.m, t.n)), N = double(std::min(t.m, t.n));
         const double fl = 4.0 * M * N * N - 4.0 * N * N * N / 3.0;
         pend_units_[QR] += fl;
@@ -328,17 +331,22 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
     // part 0: global scratch, parts 1..3: LDS classes (largest first),
     // parts 4..6: register-resident SVD with 24 / 16 / 8 rows per lane,
-    // part 7: blocked QR
-    std::vector<D> parts[8];
-    double units[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t need[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    static const int part_rpt[8] = {0, 0, 0, 0, 24, 16, 8, 1};
+    // part 7: blocked QR, part 8: CholeskyQR2
+    constexpr int NPART = 9;
+    std::vector<D> parts[NPART];
+    double units[NPART] = {};
+    int64_t need[NPART] = {};
+    static const int part_rpt[NPART] = {0, 0, 0, 0, 24, 16, 8, 1, 2};
     for (size_t i : order) {
         const int64_t bytes = doubles(tasks[i].m, tasks[i].n) * 8;
         const int64_t rows = std::max<int64_t>(tasks[i].m, tasks[i].n);
         int p = 0;
+        bool chol = false;
+        if constexpr (std::is_same<D, QrDev>::value) chol = qr_uses_chol(tasks[i].m, tasks[i].n, tasks[i].lq);
         if (kind == SVD && fast_svd_ && bytes > LDS_CLASS[1] && rows <= SVD_REG_MAX_ROWS) {
             p = rows <= 64 ? 6 : (rows <= 128 ? 5 : 4);
+        } else if (chol) {
+            p = 8;
         } else if (kind == QR && bytes > lds_max && rows <= QB_MAX_ROWS) {
             p = 7;
         } else if (bytes <= lds_max) {
@@ -348,7 +356,7 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
         units[p] += cost[i];
         need[p] = std::max(need[p], bytes);
     }
-    for (int part = 0; part < 8; part++) {
+    for (int part = 0; part < NPART; part++) {
         auto& v = parts[part];
         if (v.empty()) continue;
         Cmd c;
@@ -366,7 +374,12 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     cost.clear();
 }
 
-void Batch::flush_qr() { flush_decomp(QR, pqr_, pqr_cost_, QR_LDS_MAX, qr_doubles); }
+namespace {
+// working set of the LDS-resident QR kernel (the block and its taus)
+int64_t qr_lds_doubles(int64_t m, int64_t n) { return m * n + std::min(m, n); }
+}  // namespace
+
+void Batch::flush_qr() { flush_decomp(QR, pqr_, pqr_cost_, QR_LDS_MAX, qr_lds_doubles); }
 
 void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch) {
     double* cur = scratch;
@@ -459,6 +472,7 @@ void Batch::run(hipStream_t st) {
             check_hip(hipEventRecord(eb, st), "timer record");
             int k = int(c.kind);
             if (c.kind == QR && c.rpt == 1) k = KernelTimer::QR_BLOCKED;
+            if (c.kind == QR && c.rpt == 2) k = KernelTimer::QR_CHOL;
             if (c.kind == SVD && c.rpt == 8) k = KernelTimer::SVD_REG8;
             if (c.kind == SVD && c.rpt == 16) k = KernelTimer::SVD_REG16;
             if (c.kind == SVD && c.rpt == 24) k = KernelTimer::SVD_REG24;
