@@ -87,6 +87,18 @@ int tns_plan_flops(const tns_plan* p, int64_t* flops);
 /* execute on a HIP stream (NULL = default stream) */
 int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const* outputs, void* workspace,
                  void* stream);
+/* tns_plan_run without any device allocation.  No counterpart in the
+ * reference (TAT allocates per op); it is for callers that cannot allocate:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The plan's workspace (at most 64 MiB) and its task records (at
+ * most 4 MiB) come from one static device array the library holds on each GPU
+ * until the process ends, instead of a caller workspace and the library's
+ * growable staging.  A plan needing more workspace fails with an error
+ * (before any device call); svd plans are refused (tns_svd_factor writes into
+ * the caller's workspace).  Nothing in the array needs to start at zero.
+ * Every call on one GPU shares that GPU's array: calls on one stream are
+ * ordered, calls that overlap on different streams of one GPU race. */
+int tns_plan_run_noalloc(tns_plan* p, int dtype, const void* const* inputs, void* const* outputs, void* stream);
 /* svd phase 1: merge + per-sector SVD, then (synchronising) the TAT truncation
  * rule (svd.hpp:430-481): remain_cut < 0 means no count limit, relative_cut
  * <= 0 means none.  Output shapes are known afterwards; tns_plan_run then

@@ -45,3 +45,25 @@ def test_svd_sweep_limit_range_checked():
     L = _lib.lib()
     assert L.tns_set_svd_max_sweeps(0) != 0 and b"sweep" in L.tns_last_error()
     assert L.tns_set_svd_max_sweeps(60) == 0
+
+
+def test_noalloc_rejects_before_any_device_call():
+    """tns_plan_run_noalloc refuses plans beyond its static workspace and svd
+    plans with a clear error, before touching a device (runs without one)."""
+    import pytest
+
+    from tnsp_amd import _lib
+    from tnsp_amd import plan as P
+    from tnsp_amd.shape import KINDS, Shape
+
+    no = KINDS["No"]
+    a = Shape.create(no, ["i", "k"], [3000, 3000])
+    b = Shape.create(no, ["k", "j"], [3000, 3000])
+    big = P.contract(a, b, [("k", "k")])
+    assert big.workspace("float64") > 64 << 20
+    with pytest.raises(RuntimeError, match="static maximum"):
+        big.run_noalloc("float64", [0, 0], [0], 0)
+    s = P.svd(Shape.create(no, ["i", "j"], [4, 4]), ["i"], "U", "V", "SU", "SV")
+    with pytest.raises(RuntimeError, match="svd plans"):
+        s.run_noalloc("float64", [0], [0, 0, 0], 0)
+    assert _lib.lib().tns_plan_run_noalloc(big.handle, 7, None, None, None) != 0

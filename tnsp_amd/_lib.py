@@ -46,6 +46,7 @@ _SIGNATURES = {
     "tns_plan_workspace": [c_vp, ctypes.c_int, c_i64p],
     "tns_plan_flops": [c_vp, c_i64p],
     "tns_plan_run": [c_vp, ctypes.c_int, c_vpp, c_vpp, c_vp, c_vp],
+    "tns_plan_run_noalloc": [c_vp, ctypes.c_int, c_vpp, c_vpp, c_vp],
     "tns_svd_factor": [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int64, ctypes.c_double, c_vp],
     "tns_plan_buffers": [c_vp, c_i64p, ctypes.POINTER(ctypes.c_int)],
     "tns_svd_sectors": [c_vp, c_i64p, c_i64p, c_i32p, ctypes.POINTER(ctypes.c_int)],

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtnsgpu.so")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["core.cpp", "runtime.cpp", "capi.cpp", "engine.cpp", "kernels.hip", "proposal.hip", "proposal_capi.cpp"]
+SOURCES = ["core.cpp", "runtime.cpp", "capi.cpp", "engine.cpp", "kernels.hip", "proposal.hip", "proposal_capi.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

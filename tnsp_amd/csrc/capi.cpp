@@ -414,13 +414,25 @@ int tns_plan_flops(const tns_plan* p, int64_t* flops) {
 
 int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const* outputs, void* workspace,
                  void* stream) {
-    TNS_TRY({
+    TNS_TRY({ tns::run_plan(p, dtype, inputs, outputs, workspace, as_stream(stream), nullptr); })
+}
+
+}  // extern "C"
+
+int64_t tns::plan_workspace_bytes(const tns_plan* p, int dtype) { return layout_of(p, dtype).total; }
+
+bool tns::plan_is_svd(const tns_plan* p) { return p->type == tns_plan::SVD; }
+
+void tns::run_plan(tns_plan* p, int dtype, const void* const* inputs, void* const* outputs, void* workspace,
+                   hipStream_t stream, Staging* staging) {
+    {
         if (dtype != 0 && dtype != 1) throw Error("unsupported dtype");
         const int64_t es = dtype == 0 ? 8 : 4;
         Layout L = layout_of(p, dtype);
         char* ws = static_cast<char*>(workspace);
         if (L.total > 0 && !ws) throw Error("workspace required");
         Batch bt(dtype);
+        if (staging) bt.set_staging(staging);
         switch (p->type) {
             case tns_plan::EDGEOP:
             case tns_plan::CONJ: {
@@ -474,9 +486,11 @@ int tns_plan_run(tns_plan* p, int dtype, const void* const* inputs, void* const*
                 break;
             }
         }
-        bt.run(as_stream(stream));
-    })
+        bt.run(stream);
+    }
 }
+
+extern "C" {
 
 // ------------------------------------------------------------- batches
 // The same ops for many tensors at once: planning spread over host threads,

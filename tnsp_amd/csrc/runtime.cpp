@@ -32,6 +32,7 @@ Staging& Staging::get() {
 }
 
 Staging::~Staging() {
+    if (fixed_) return;
     if (pending_ && ev_) (void)hipEventSynchronize(ev_);
     if (in_use_ && done_) (void)hipEventSynchronize(done_);
     if (ev_) (void)hipEventDestroy(ev_);
@@ -43,6 +44,14 @@ Staging::~Staging() {
 char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
     const size_t n = blob.size();
     if (n == 0) return dev_;
+    if (fixed_) {
+        if (n > dev_cap_)
+            throw Error("task records (" + std::to_string(n) + " bytes) exceed the fixed staging buffer (" +
+                        std::to_string(dev_cap_) + " bytes)");
+        // pageable source: the copy has consumed blob when the call returns
+        check_hip(hipMemcpyAsync(dev_, blob.data(), n, hipMemcpyHostToDevice, st), "task upload");
+        return dev_;
+    }
     if (!ev_) check_hip(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "event create");
     if (!done_) check_hip(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "event create");
     // the pinned staging buffer may still feed the previous copy
@@ -75,7 +84,7 @@ char* Staging::upload(const std::vector<char>& blob, hipStream_t st) {
 }
 
 void Staging::release(hipStream_t st) {
-    if (!done_) return;
+    if (fixed_ || !done_) return;
     check_hip(hipEventRecord(done_, st), "staging release");
     in_use_ = true;
     last_ = st;

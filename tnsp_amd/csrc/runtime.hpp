@@ -28,6 +28,11 @@ class Staging {
    public:
     static Staging& get();
     Staging() = default;
+    // fixed mode: task records go to the given device buffer of cap bytes,
+    // copied straight from the host blob; no memory or event is ever
+    // allocated, and an upload larger than cap throws.  Uploads on one stream
+    // are ordered; uploads on different streams into the same buffer race.
+    Staging(char* fixed_dev, size_t cap) : dev_(fixed_dev), dev_cap_(cap), fixed_(true) {}
     Staging(const Staging&) = delete;
     Staging& operator=(const Staging&) = delete;
     ~Staging();
@@ -37,6 +42,7 @@ class Staging {
     void release(hipStream_t st);
 
    private:
+    bool fixed_ = false;
     char* dev_ = nullptr;
     size_t dev_cap_ = 0;
     char* host_ = nullptr;

@@ -98,6 +98,14 @@ class Plan:
         _lib.check(_lib.lib().tns_plan_run(self.handle, DTYPE_CODE[dtype], ins, outs, ctypes.c_void_p(int(workspace)),
                                            ctypes.c_void_p(int(stream))))
 
+    def run_noalloc(self, dtype, inputs, outputs, stream):
+        """run() with the workspace and task records in the library's static
+        device array (tns_plan_run_noalloc): no device allocation."""
+        ins = (ctypes.c_void_p * max(1, len(inputs)))(*[int(p) for p in inputs])
+        outs = (ctypes.c_void_p * max(1, len(outputs)))(*[int(p) for p in outputs])
+        _lib.check(_lib.lib().tns_plan_run_noalloc(self.handle, DTYPE_CODE[dtype], ins, outs,
+                                                   ctypes.c_void_p(int(stream))))
+
     def svd_factor(self, dtype, inp, workspace, remain_cut, relative_cut, stream):
         _lib.check(_lib.lib().tns_svd_factor(self.handle, DTYPE_CODE[dtype], ctypes.c_void_p(int(inp)),
                                              ctypes.c_void_p(int(workspace)), int(remain_cut), float(relative_cut),
