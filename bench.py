@@ -11,6 +11,12 @@ the dominant kernel measured with HIP events on the launch stream during the
 timed steps, and the reference TAT (header-only C++ built from the reference
 sources into oracle/_ref) timed on this host's cores on a bounded sample of
 the same workload.
+
+Kernel durations come from a separate pass outside the timed region: one
+chunk of the last timed batch on one lane with HIP events on the launch
+stream (events under concurrent lanes would also time the queueing behind
+the other lanes' kernels), so they agree with a rocprofv3 kernel trace of
+a one-lane run.
 """
 import argparse
 import json
@@ -51,6 +57,9 @@ def parse():
     ap.add_argument("--ess-engine-sweep", action="store_true",
                     help="also measure the sweep that recomputes psi(s') with the amplitude engine")
     ap.add_argument("--ess-samples", type=int, default=16)
+    ap.add_argument("--ess-default", default="C4",
+                    help="workload of the bounded ESS leg of the default run (learned proposal vs sweep)")
+    ap.add_argument("--no-ess", action="store_true", help="skip the ESS leg of the default run")
     ap.add_argument("--ess-train-sweeps", type=int, default=6)
     ap.add_argument("--ess-epochs", type=int, default=6)
     ap.add_argument("--ess-learned-k", default="",
@@ -178,7 +187,8 @@ def sweep_timing(args, log, eng=None):
             "reference_chain": args.ess_diagonal == "hole",
             "sweep_s": dt, "chain_sweeps_per_s": len(start) / dt, "bonds": len(smp.bonds),
             "proposals_per_s": smp.proposals / (args.sweep_count + 1) / dt,
-            "acceptance": smp.accepted / max(1, smp.proposals), "hole_refreshes": smp.refreshed, "observer_s": de,
+            "acceptance": smp.accepted / max(1, smp.proposals), "hole_refreshes": smp.refreshed,
+            "respun_chains": smp.respun, "observer_s": de,
             "energy_per_site": float(np.nanmean(e) / (spec.L1 * spec.L2))}
 
 
@@ -213,9 +223,10 @@ def host_cores():
 
 
 def kernel_rates(kstats):
-    """Achieved rate of each kernel kind over the timed launches (HIP events
-    on the launch stream): flops for GEMM / QR / SVD against the fp64 matrix
-    peak, bytes for permute / fill / diag against HBM."""
+    """Achieved rate of each kernel kind over the launches of the kernel
+    timing pass (HIP events on the launch stream): flops for GEMM / QR / SVD
+    against the fp64 matrix peak, bytes for permute / fill / diag against
+    HBM."""
     out = {}
     for kind, s in kstats.items():
         if s["launches"] == 0 or s["ms"] <= 0:
@@ -447,7 +458,6 @@ def main():
         eng.evaluate(configs[w * B:(w + 1) * B])
         print(f"[bench] warmup step {w} {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
     eng.reset_stats()
-    eng.set_profile(True)
     torch.cuda.synchronize()
     t0 = time.time()
     psis = []
@@ -457,11 +467,21 @@ def main():
         print(f"[bench] step {s} done {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.time() - t0
-    eng.set_profile(False)
     psi = np.concatenate(psis)
     st = eng.stats()
-    ks = eng.kernel_stats()
     value = B * args.steps / elapsed
+    # kernel durations: one chunk on one lane, outside the timed region
+    i0 = (args.warmup + args.steps - 1) * B
+    eng.set_lanes(1)
+    eng.reset_stats()
+    eng.set_profile(True)
+    eng.evaluate(configs[i0:i0 + min(B, args.chunk)])
+    torch.cuda.synchronize()
+    eng.set_profile(False)
+    ks = eng.kernel_stats()
+    eng.set_lanes(args.lanes)
+    print(f"[bench] kernel timing pass: {min(B, args.chunk)} configurations on one lane", file=sys.stderr,
+          flush=True)
     dom = dominant_kernel(ks)
     roof = dom[1] if dom else None
     if roof:
@@ -507,6 +527,21 @@ def main():
                                      else None)
         except Exception as e:  # noqa: BLE001
             env_sweep = {"error": f"{type(e).__name__}: {e}"}
+    ess_leg = None
+    if not args.ess and not args.no_ess and args.ess_default:
+        # bounded ESS leg outside the timed region: 128 chains, 8 samples
+        # after 2 burn-in, learned proposal trained on 2 sweeps' hops, K = 4
+        # proposals per learned sample
+        try:
+            sub = argparse.Namespace(**vars(args))
+            sub.ess, sub.ess_chains, sub.ess_samples = args.ess_default, 128, 8
+            sub.ess_train_sweeps, sub.ess_epochs, sub.ess_learned_k = 2, 4, "4"
+            sub.ess_engine_sweep = False
+            t0 = time.time()
+            ess_leg = ess_comparison(sub, log)
+            ess_leg["wall_s"] = time.time() - t0
+        except Exception as e:  # noqa: BLE001
+            ess_leg = {"error": f"{type(e).__name__}: {e}"}
     direct = None
     if not args.no_direct:
         # direct sampling on the 4x4 workload, also outside the timed region
@@ -541,7 +576,17 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity_gate": gate,
-        "extra": {"kernel_ms": {k: v["ms"] for k, v in ks.items()},
+        "ess_per_s": None if not (ess or ess_leg) or "error" in (ess or ess_leg) else {
+            "workload": (ess or ess_leg)["workload"], "chains": (ess or ess_leg)["chains"],
+            "sweep": (ess or ess_leg)["sweep"]["ess_per_s"],
+            "learned": max(r["ess_per_s"] for r in (ess or ess_leg)["learned"]),
+            "gain": (ess or ess_leg)["gain"], "gain_second_observable": (ess or ess_leg)["gain_density"],
+            "reference_chain": (ess or ess_leg)["sweep"]["reference_chain"],
+            "note": "ESS of the tV energy estimator per second of sampling (t = V = 1); the sweep is the "
+                    "cached-environment replace sweep; reference_chain false: doubly occupied bonds keep "
+                    "psi instead of tetragono's hole(()) refresh"},
+        "extra": {"kernel_timing": f"one lane, {min(B, args.chunk)} configurations, HIP events on the launch stream",
+                  "kernel_ms": {k: v["ms"] for k, v in ks.items()},
                   "kernel_rates": kernel_rates(ks),
                   "kernel_launches": {k: v["launches"] for k, v in ks.items()},
                   "host_plan_s": st["plan_s"], "host_bind_s": st["bind_s"], "svals_wait_s": st["sync_s"],
@@ -550,7 +595,8 @@ def main():
                   "svd_gflop": st["svd_flops"] / 1e9, "perm_gbytes": st["perm_elems"] * 16 / 1e9,
                   "device_bytes": st["device_bytes"], "nonzero_fraction": nonzero,
                   "plain_path_configs_per_s": plain,
-                  "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned, "ess": ess,
+                  "vs_plain_path": (value / plain) if plain else None, "precision": args.precision, "gemm_tiles": tuned,
+                  "ess": ess or ess_leg,
                   "environment_sweep": env_sweep, "direct_sampling": direct},
     }
     print(json.dumps(line), flush=True)

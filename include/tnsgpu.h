@@ -178,7 +178,7 @@ int tns_engine_stats(const tns_engine* e, double* values, int n);
 int tns_engine_reset_stats(tns_engine* e);
 /* per-kernel device timing with HIP events on the launch stream: permute,
  * fill, gemm, qr (LDS / unblocked), svd (LDS / global), diag, qr_blocked,
- * svd_reg8, svd_reg16, svd_reg24, qr_chol; per kernel: launches, total ms,
+ * svd_reg8, svd_reg16, svd_reg24; per kernel: launches, total ms,
  * algorithmic work (bytes for permute/fill/diag, flops for the others) */
 int tns_engine_set_profile(tns_engine* e, int on);
 /* the caller changed the site tensors' values in place (same buffers and

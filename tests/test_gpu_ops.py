@@ -263,10 +263,12 @@ def test_transpose_large_dims_bit_exact():
 @pytest.mark.parametrize("dtype,tol", [("float64", 1e-12), ("float32", 2e-5)])
 @pytest.mark.parametrize("rank_deficient", [False, True])
 @pytest.mark.parametrize("cols", [177, 96, 20])
-def test_qr_tall_cholesky_path(dtype, tol, rank_deficient, cols):
-    """Tall blocks beyond LDS (1200 rows) take the CholeskyQR2 kernel; a block
-    with two equal columns breaks its Cholesky and takes the Householder
-    body instead.  Either way Q R = A, Q^T Q = 1 and R is upper triangular."""
+@pytest.mark.parametrize("order", ["ab", "ba"])
+def test_qr_tall_beyond_lds(dtype, tol, rank_deficient, cols, order):
+    """Tall blocks beyond LDS (1200 rows) take the blocked Householder
+    kernel, stored either way round (order "ba" stores the block transposed,
+    which TAT factors as an LQ of the wide matrix), with two equal columns
+    (rank-deficient) or not: Q R = A, Q^T Q = 1 and R is upper triangular."""
     from tnsp_amd import TAT
 
     rng = np.random.default_rng(cols)
@@ -274,11 +276,45 @@ def test_qr_tall_cholesky_path(dtype, tol, rank_deficient, cols):
     if rank_deficient:
         x[:, cols // 2] = x[:, 1]
     mod = TAT.No.D if dtype == "float64" else TAT.No.S
-    a = mod.Tensor(["a", "b"], [1200, cols])
-    a.storage = x.reshape(-1)
+    if order == "ab":
+        a = mod.Tensor(["a", "b"], [1200, cols])
+        a.storage = x.reshape(-1)
+    else:
+        a = mod.Tensor(["b", "a"], [cols, 1200])
+        a.storage = x.T.reshape(-1)
     q, r = a.qr("r", {"b"}, "Q", "R")
     qm = np.asarray(storage(q.transpose(["a", "Q"])), dtype=np.float64).reshape(1200, cols)
     rm = np.asarray(storage(r.transpose(["R", "b"])), dtype=np.float64).reshape(cols, cols)
     assert np.abs(qm @ rm - x).max() < tol * np.abs(x).max() * 10
     assert np.abs(qm.T @ qm - np.eye(cols)).max() < tol * 10
     assert np.array_equal(np.tril(rm, -1), np.zeros_like(rm))
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e-17, 1e-150])
+@pytest.mark.parametrize("rows,cols,rank", [(4, 9, 3), (9, 4, 3), (40, 130, 35), (150, 170, 120), (300, 200, 150)])
+def test_svd_structurally_rank_deficient(scale, rows, cols, rank):
+    """Blocks whose columns span fewer dimensions than their count (zero
+    columns of M, as the 8x8 PEPS produces at some configurations) and blocks
+    at 1e-150: every Jacobi kernel (LDS, register, global) converges, the
+    singular values equal LAPACK's to 1e-13 of the largest and U S V
+    rebuilds the block.  The null-space columns used to keep rotating into
+    the underflow range until the sweep limit."""
+    from tnsp_amd import TAT
+
+    rng = np.random.default_rng(rows + cols)
+    x = np.zeros((rows, cols))
+    x[:, :rank] = rng.standard_normal((rows, rank)) * scale
+    a = TAT.No.D.Tensor(["a", "b"], [rows, cols])
+    a.storage = x.reshape(-1)
+    u, s, v = a.svd({"a"}, "U", "V", "SU", "SV")
+    # exact zeros are dropped, as TAT's rule keeps only values above zero
+    k = s.edges[0].dimension()
+    assert rank <= k <= min(rows, cols)
+    um = np.asarray(storage(u.transpose(["a", "U"]))).reshape(rows, k)
+    sm = np.asarray(storage(s.transpose(["SU", "SV"]))).reshape(k, k)
+    vm = np.asarray(storage(v.transpose(["V", "b"]))).reshape(k, cols)
+    ref = np.linalg.svd(x, compute_uv=False)
+    got = np.sort(np.abs(np.diag(sm)))[::-1]
+    assert np.abs(got[:rank] - ref[:rank]).max() <= 1e-13 * ref[0]
+    assert np.all(got[rank:] <= 1e-13 * ref[0])
+    assert np.abs(um @ sm @ vm - x).max() <= 1e-12 * np.abs(x).max()

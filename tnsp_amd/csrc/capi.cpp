@@ -138,7 +138,7 @@ struct Layout {
 };
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-int64_t qr_scratch(const QrTask& t) { return qr_doubles(t.m, t.n, t.lq); }
+int64_t qr_scratch(const QrTask& t) { return qr_doubles(t.m, t.n); }
 int64_t svd_scratch(const SvdTask& t) { return svd_doubles(t.m, t.n); }
 
 Layout layout_of(const tns_plan* p, int dtype) {

@@ -58,7 +58,8 @@ void Pool::work(int id) {
 
 void Pool::parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& fn) {
     if (n <= 0) return;
-    if (workers_.empty() || n < 4) {
+    // waking the workers costs more than planning a few items
+    if (workers_.empty() || n < 16) {
         fn(0, n);
         return;
     }
@@ -313,7 +314,7 @@ void Engine::qr(std::vector<Slot*>& t, const std::vector<int32_t>& free_names, i
         t1[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t1.size * es_)));
         t2[i] = int64_t(arenas_[A_TEMP].alloc(size_t(p.t2.size * es_)));
         int64_t sc = 0;
-        for (auto& task : p.qrs) sc += qr_doubles(task.m, task.n, task.lq);
+        for (auto& task : p.qrs) sc += qr_doubles(task.m, task.n);
         ts[i] = int64_t(arenas_[A_TEMP].alloc(size_t(sc * 8)));
         oq[i] = int64_t(arenas_[oa].alloc(size_t(p.q.size * es_)));
         orr[i] = int64_t(arenas_[oa].alloc(size_t(p.r.size * es_)));

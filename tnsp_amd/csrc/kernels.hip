@@ -12,6 +12,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace tns {
 
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict
 }
 
 // --------------------------------------------------------- reductions
-constexpr int DPP_ROW_SHL1 = 0x101, DPP_ROW_SHR1 = 0x111, DPP_ROW_ROR8 = 0x128;
+constexpr int DPP_ROW_ROR1 = 0x121, DPP_ROW_ROR2 = 0x122, DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
 
 // a double moved across lanes by a DPP row shift (two 32-bit halves)
 template<int CTRL>
@@ -448,17 +449,79 @@ __device__ inline double sum8(double v) {
     return swap_sum<32>(v);
 }
 
-__device__ inline double wave_sum(double v) {
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+// sum over the 16 lanes of a DPP row by rotations 8, 4, 2, 1: after the
+// step with rotation r every lane holds the sum of its 16/r-lane orbit, and
+// the two lanes of each addition hold the same two values, so every lane of
+// the row ends with bitwise the same sum (IEEE addition commutes)
+__device__ inline double row_sum16(double v) {
+    v += dpp_f64<DPP_ROW_ROR8>(v);
+    v += dpp_f64<DPP_ROW_ROR4>(v);
+    v += dpp_f64<DPP_ROW_ROR2>(v);
+    v += dpp_f64<DPP_ROW_ROR1>(v);
     return v;
 }
 
-// butterfly sum within aligned groups of G lanes (G = 16, 32 or 64): every
-// lane of the group ends with the same, order-independent value
+// sum within aligned groups of G lanes (G = 16, 32 or 64), VALU only: every
+// lane of the group ends with bitwise the same value
 __device__ inline double group_sum(double v, int G) {
-    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    v = row_sum16(v);
+    if (G >= 32) v = swap_sum<16>(v);
+    if (G >= 64) v = swap_sum<32>(v);
     return v;
 }
+
+__device__ inline double wave_sum(double v) { return group_sum(v, WAVE); }
+
+// maximum over the wave (rotations within rows, then the row / half swaps)
+__device__ inline double wave_max(double v) {
+    v = fmax(v, dpp_f64<DPP_ROW_ROR8>(v));
+    v = fmax(v, dpp_f64<DPP_ROW_ROR4>(v));
+    v = fmax(v, dpp_f64<DPP_ROW_ROR2>(v));
+    v = fmax(v, dpp_f64<DPP_ROW_ROR1>(v));
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(u), hi = unsigned(u >> 32);
+    {
+        const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        const double a = __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(h[0])) << 32) | unsigned(l[0]));
+        const double b = __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(h[1])) << 32) | unsigned(l[1]));
+        v = fmax(a, b);
+    }
+    const unsigned long long u2 = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo2 = unsigned(u2), hi2 = unsigned(u2 >> 32);
+    const auto l = __builtin_amdgcn_permlane32_swap(lo2, lo2, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi2, hi2, false, false);
+    const double a = __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(h[0])) << 32) | unsigned(l[0]));
+    const double b = __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(h[1])) << 32) | unsigned(l[1]));
+    return fmax(a, b);
+}
+
+// Scale exponent of a block for the Jacobi SVDs: the block is multiplied by
+// 2^-e (exact) so that its largest entry lies in [1, 2); squared column norms
+// then neither overflow nor underflow for every column that matters.  0 for
+// an all-zero or non-finite block.
+__device__ inline int jacobi_scale_exponent(double amax) {
+    return (amax > 0.0 && amax <= 1.79e308) ? ilogb(amax) : 0;
+}
+
+// Squared column norm (after the scaling) at or below which a column counts
+// as zero: it takes part in no rotation and gets singular value 0.  Its true
+// singular value is below 1e-100 of the largest entry.  A column of an
+// exactly rank-deficient block shrinks by a rounding factor with every
+// rotation against the columns spanning its space, and would otherwise keep
+// rotating down into the underflow range, where its relative cosines stay
+// of order one and the sweeps never end.
+constexpr double JACOBI_QUIET = 1e-200;
+
+// A column whose squared norm falls to JACOBI_DEAD2 times the block's squared
+// Frobenius norm (1e-15 of the norm, about 4.5 ulp) is rounding noise: it
+// takes part in no further rotation.  The columns of an exactly
+// rank-deficient block that shrink into the null space are noise inside the
+// span of the others, so their cosines with those stay of order one; they
+// shrink only slowly as they mix among themselves, and would otherwise keep
+// the sweeps going for dozens of sweeps or forever.  Their singular values
+// come out at that noise level, as LAPACK's do.
+constexpr double JACOBI_DEAD2 = 1e-30;
 
 template<int NT>
 __device__ inline double block_sum(double v, double* red) {
@@ -474,6 +537,33 @@ __device__ inline double block_sum(double v, double* red) {
 
 // lanes per group for work along a column of length len
 __device__ inline int group_lanes(int len) { return len <= 48 ? 16 : (len <= 96 ? 32 : 64); }
+
+// Jacobi rotation (cs, sn) zeroing the off-diagonal of [[saa, sab], [sab,
+// sbb]] (Rutishauser's formulas, sab != 0).  The two reciprocals after the
+// exact zeta use the hardware reciprocal / reciprocal square root refined by
+// two Newton steps each: their arguments lie in [1, inf), and
+// cs^2 + sn^2 = cs^2 (1 + t^2) stays 1 to rounding, which is what keeps the
+// accumulated transformation orthogonal.
+__device__ inline double refined_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+__device__ inline double refined_rsqrt(double y) {
+    double r = __builtin_amdgcn_rsq(y);
+    r = r * fma(-0.5 * y * r, r, 1.5);
+    return r * fma(-0.5 * y * r, r, 1.5);
+}
+
+__device__ inline void jacobi_rotation(double saa, double sbb, double sab, double& cs, double& sn) {
+    const double zeta = (sbb - saa) / (2.0 * sab);
+    // for |zeta| beyond 1e150, zeta^2 would overflow: t = 1 / (2 zeta) there
+    const double t = fabs(zeta) > 1e150 ? 0.5 / zeta
+                                        : copysign(refined_rcp(fabs(zeta) + sqrt(fma(zeta, zeta, 1.0))), zeta);
+    cs = refined_rsqrt(fma(t, t, 1.0));
+    sn = cs * t;
+}
 
 // --------------------------------------------------------------------- qr
 // Householder QR of X (mx x nx, column-major), X = M (qr) or X = M^T (lq);
@@ -872,296 +962,6 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
     }
 }
 
-// ----------------------------------------------------------------- cholqr
-// CholeskyQR2 for tall QR sectors beyond LDS (M = Q R with m >= n, n <=
-// 16 * QC_T): one workgroup of QR_WG threads per sector, every product on the
-// fp64 MFMA (16x16x4):
-//   G = M^T M                  (upper 16x16 tiles, spread over the waves,
-//                               accumulated in registers)
-//   G = R1^T R1                (tiled right-looking Cholesky, tiles kept in
-//                               registers; each diagonal tile factored and
-//                               inverted in LDS by one wave)
-//   Q1 = M R1^-1               (block forward substitution over the column
-//                               tiles, 16 rows per wave at a time)
-// then the same on Q1 (R2, Q = Q1 R2^-1) and R = R2 R1.  M is read twice, Q1
-// written once and read twice: the traffic of about six passes over the
-// block instead of two per Householder panel.  Orthogonality and residual are
-// at rounding level while cond(M) is far below 1e8; a Cholesky pivot below
-// 1e-10 of its column's squared norm (or not positive) sends the whole sector
-// to the Householder kernel body instead, so rank-deficient blocks still get
-// LAPACK's answer.  Scratch: Q1 (m n) and the R1 / R2 tiles and the inverses
-// of the diagonal tiles (QC_EXTRA doubles).
-constexpr int QC_NW = QR_WG / WAVE;
-constexpr int QC_NTL = QC_T * (QC_T + 1) / 2;
-constexpr int QC_SLOTS = (QC_NTL + QC_NW - 1) / QC_NW;
-
-// upper tile t (row-major over I <= J) -> I, J
-__host__ __device__ constexpr int qc_tile_i(int t) {
-    int i = 0;
-    while (t >= QC_T - i) {
-        t -= QC_T - i;
-        i++;
-    }
-    return i;
-}
-__host__ __device__ constexpr int qc_tile_j(int t) {
-    int i = 0;
-    while (t >= QC_T - i) {
-        t -= QC_T - i;
-        i++;
-    }
-    return i + t;
-}
-__host__ __device__ constexpr int qc_tile(int i, int j) { return i * QC_T - i * (i - 1) / 2 + (j - i); }
-
-__device__ inline d4 qc_mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-
-// acc[t / QC_NW] of wave t % QC_NW = tile t of X^T X (X: m x n row-major)
-template<typename S>
-__device__ inline void qc_gram(const S* __restrict__ X, int m, int n, int nt, d4* acc) {
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-#pragma unroll
-    for (int s = 0; s < QC_SLOTS; s++) acc[s] = d4{0.0, 0.0, 0.0, 0.0};
-    const int c = lane % 16;
-    for (int k0 = 0; k0 < m; k0 += 4) {
-        const int r = k0 + lane / 16;
-        double b[QC_T];
-#pragma unroll
-        for (int J = 0; J < QC_T; J++) {
-            const int col = 16 * J + c;
-            b[J] = (J < nt && r < m && col < n) ? double(X[int64_t(r) * n + col]) : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < QC_NTL; t++) {
-            if (t % QC_NW == wave && qc_tile_j(t) < nt)
-                acc[t / QC_NW] = qc_mfma(b[qc_tile_i(t)], b[qc_tile_j(t)], acc[t / QC_NW]);
-        }
-    }
-}
-
-// In-place tiled Cholesky of the register tiles (acc: G -> R, upper); R
-// tiles also go row-major to Rg (tile (I, J) at (I QC_T + J) 256), the
-// inverses of the diagonal tiles to RIg (tile J at J 256).  Sets *fail when a
-// pivot falls below 1e-10 of its column's squared norm.
-__device__ inline void qc_cholesky(d4* acc, int n, int nt, double* __restrict__ Rg, double* __restrict__ RIg,
-                                   double (*D)[17], double (*E)[17], double (*Rrow)[16][16], double* gdiag,
-                                   int* fail) {
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    // squared column norms (diagonal of G) for the breakdown test
-#pragma unroll
-    for (int K = 0; K < QC_T; K++) {
-        const int t = qc_tile(K, K);
-        if (K < nt && t % QC_NW == wave) {
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                if (lane / 16 + 4 * i == lane % 16) gdiag[16 * K + lane % 16] = acc[t / QC_NW][i];
-        }
-    }
-#pragma unroll
-    for (int K = 0; K < QC_T; K++) {
-        if (K >= nt) break;
-        const int tkk = qc_tile(K, K);
-        if (tkk % QC_NW == wave) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) D[lane / 16 + 4 * i][lane % 16] = acc[tkk / QC_NW][i];
-        }
-        __syncthreads();
-        if (wave == 0) {
-            // unblocked Cholesky of the 16x16 tile (upper R with R^T R = D)
-            for (int c = 0; c < 16; c++) {
-                if (lane == 0) {
-                    // columns past n (zero padding of the last tile) get a
-                    // unit pivot: their rows and columns of G are zero
-                    const double p = D[c][c];
-                    const double g = gdiag[16 * K + c];
-                    const bool pad = 16 * K + c >= n;
-                    if (!pad && (!(p > 1e-10 * g) || !(g > 0.0))) *fail = 1;
-                    D[c][c] = (pad || !(p > 0.0)) ? 1.0 : sqrt(p);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                if (lane > c && lane < 16) D[c][lane] /= D[c][c];
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                for (int e = lane; e < 256; e += WAVE) {
-                    const int i = e / 16, j = e % 16;
-                    if (i > c && j >= i) D[i][j] -= D[c][i] * D[c][j];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-            }
-            for (int e = lane; e < 256; e += WAVE) {
-                const int i = e / 16, j = e % 16;
-                if (i > j) D[i][j] = 0.0;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            // E = D^-1 (upper), column j by lane j, back substitution
-            if (lane < 16) {
-                const int j = lane;
-                for (int i = 15; i >= 0; i--) {
-                    double v;
-                    if (i > j) {
-                        v = 0.0;
-                    } else if (i == j) {
-                        v = 1.0 / D[j][j];
-                    } else {
-                        double sacc = 0.0;
-                        for (int l = i + 1; l <= j; l++) sacc += D[i][l] * E[l][j];
-                        v = -sacc / D[i][i];
-                    }
-                    E[i][j] = v;
-                }
-            }
-        }
-        __syncthreads();
-        // row K: R_KK from D, R_KJ = R_KK^-T G_KJ (A = E^T, B = the tile)
-#pragma unroll
-        for (int J = 0; J < QC_T; J++) {
-            if (J < K) continue;
-            const int t = qc_tile(K, J);
-            if (J >= nt || t % QC_NW != wave) continue;
-            d4 r;
-            if (J == K) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) r[i] = D[lane / 16 + 4 * i][lane % 16];
-#pragma unroll
-                for (int i = 0; i < 4; i++) RIg[K * 256 + 64 * i + lane] = E[lane / 16 + 4 * i][lane % 16];
-            } else {
-                r = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int s = 0; s < 4; s++) r = qc_mfma(E[4 * s + lane / 16][lane % 16], acc[t / QC_NW][s], r);
-            }
-            acc[t / QC_NW] = r;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                Rrow[J][lane / 16 + 4 * i][lane % 16] = r[i];
-                Rg[(K * QC_T + J) * 256 + 64 * i + lane] = r[i];
-            }
-        }
-        __syncthreads();
-        // trailing tiles (I, J), K < I <= J: G_IJ -= R_KI^T R_KJ
-#pragma unroll
-        for (int t = 0; t < QC_NTL; t++) {
-            const int I = qc_tile_i(t), J = qc_tile_j(t);
-            if (I <= K || J >= nt || t % QC_NW != wave) continue;
-#pragma unroll
-            for (int s = 0; s < 4; s++)
-                acc[t / QC_NW] = qc_mfma(-Rrow[I][4 * s + lane / 16][lane % 16], Rrow[J][4 * s + lane / 16][lane % 16],
-                                         acc[t / QC_NW]);
-        }
-    }
-    __syncthreads();
-}
-
-// Q = X R^-1 by block forward substitution over the column tiles: per 16-row
-// block, tile J of Q^T = R_JJ^-T (X_J^T - sum_{I<J} R_IJ^T Q_I^T), the tiles of
-// Q^T in MFMA accumulator layout (which is the B-operand layout of the next
-// products)
-template<typename S, typename TO>
-__device__ inline void qc_solve(const S* __restrict__ X, int m, int n, int nt, const double* __restrict__ Rg,
-                                const double* __restrict__ RIg, TO* __restrict__ Q) {
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    const int nrb = (m + 15) / 16;
-    for (int rb = wave; rb < nrb; rb += QC_NW) {
-        const int r = 16 * rb + lane % 16;
-        d4 q[QC_T];
-#pragma unroll
-        for (int J = 0; J < QC_T; J++) {
-            if (J >= nt) break;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int col = 16 * J + lane / 16 + 4 * i;
-                q[J][i] = (r < m && col < n) ? double(X[int64_t(r) * n + col]) : 0.0;
-            }
-#pragma unroll
-            for (int I = 0; I < QC_T; I++) {
-                if (I >= J) break;
-                const double* rt = Rg + (I * QC_T + J) * 256;
-#pragma unroll
-                for (int s = 0; s < 4; s++) q[J] = qc_mfma(-rt[64 * s + lane], q[I][s], q[J]);
-            }
-            d4 o = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int s = 0; s < 4; s++) o = qc_mfma(RIg[J * 256 + 64 * s + lane], q[J][s], o);
-            q[J] = o;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int col = 16 * J + lane / 16 + 4 * i;
-                if (r < m && col < n) Q[int64_t(r) * n + col] = TO(o[i]);
-            }
-        }
-    }
-}
-
-template<typename T>
-__global__ __launch_bounds__(QR_WG) void qr_kernel_chol(const QrDev* __restrict__ tasks) {
-    __shared__ double D[16][17];
-    __shared__ double E[16][17];
-    __shared__ double Rrow[QC_T][16][16];
-    __shared__ double gdiag[16 * QC_T];
-    __shared__ double red[QR_WG / WAVE];
-    __shared__ double bc[2];
-    __shared__ int fail;
-    const QrDev tk = tasks[blockIdx.x];
-    const int m = tk.m, n = tk.n, nt = (n + 15) / 16;
-    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
-    if (tk.lq || m < n || nt > QC_T) {
-        // the host routes only fitting blocks here
-        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
-        return;
-    }
-    double* Q1 = tk.work;
-    double* R1 = Q1 + int64_t(m) * n;
-    double* RI = R1 + QC_T * QC_T * 256;
-    double* R2 = RI + QC_T * 256;
-    const T* M = static_cast<const T*>(tk.in);
-    if (threadIdx.x == 0) fail = 0;
-    __syncthreads();
-    d4 acc[QC_SLOTS];
-    qc_gram(M, m, n, nt, acc);
-    qc_cholesky(acc, n, nt, R1, RI, D, E, Rrow, gdiag, &fail);
-    if (fail) {
-        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
-        return;
-    }
-    qc_solve(M, m, n, nt, R1, RI, Q1);
-    __syncthreads();
-    qc_gram(Q1, m, n, nt, acc);
-    qc_cholesky(acc, n, nt, R2, RI, D, E, Rrow, gdiag, &fail);
-    if (fail) {
-        qr_body<T>(tk, tk.work, tk.work + int64_t(m) * n, red, bc);
-        return;
-    }
-    T* Qo = static_cast<T*>(tk.f1);
-    T* Ro = static_cast<T*>(tk.f2);
-    qc_solve(Q1, m, n, nt, R2, RI, Qo);
-    // R = R2 R1 (upper tiles), lower tiles zero; R is n x n row-major
-#pragma unroll
-    for (int t = 0; t < QC_NTL; t++) {
-        const int I = qc_tile_i(t), J = qc_tile_j(t);
-        if (J >= nt || t % QC_NW != wave) continue;
-        d4 r = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int L = 0; L < QC_T; L++) {
-            if (L < I || L > J) continue;
-            const double* a = R2 + (I * QC_T + L) * 256;
-            const double* b = R1 + (L * QC_T + J) * 256;
-#pragma unroll
-            for (int s = 0; s < 4; s++) r = qc_mfma(a[(lane % 16) * 16 + 4 * s + lane / 16], b[64 * s + lane], r);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = 16 * I + lane / 16 + 4 * i, col = 16 * J + lane % 16;
-            if (row < n && col < n) Ro[int64_t(row) * n + col] = T(r[i]);
-        }
-    }
-    for (int e = threadIdx.x; e < n * n; e += QR_WG) {
-        const int row = e / n, col = e % n;
-        if (row / 16 > col / 16) Ro[e] = T(0);
-    }
-}
-
 // -------------------------------------------------------------------- svd
 // Ranks the singular values (descending, ties by column) and writes S, U
 // and V^T of the block from X (normalised columns), V and sig.
@@ -1207,14 +1007,39 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
     const T* in = static_cast<const T*>(tk.in);
     const int tid = threadIdx.x;
     const int G = group_lanes(mx), NG = SVD_WG / G, g = tid / G, gl = tid % G;
+    __shared__ double wmax[SVD_WG / WAVE];
+    double amax = 0.0;
     for (int64_t e = tid; e < int64_t(mx) * kx; e += SVD_WG) {
         const int r = int(e % mx), c = int(e / mx);
-        X[e] = double(trans ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
+        const double x = double(trans ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
+        X[e] = x;
+        amax = fmax(amax, fabs(x));
     }
     for (int64_t e = tid; e < int64_t(kx) * kx; e += SVD_WG) {
         const int r = int(e % kx), c = int(e / kx);
         V[e] = (r == c) ? 1.0 : 0.0;
     }
+    amax = wave_max(amax);
+    if (tid % WAVE == 0) wmax[tid / WAVE] = amax;
+    __syncthreads();
+    for (int w = 0; w < SVD_WG / WAVE; w++) amax = fmax(amax, wmax[w]);
+    const int ex = jacobi_scale_exponent(amax);
+    const double scl = ldexp(1.0, -ex);
+    for (int64_t e = tid; e < int64_t(mx) * kx; e += SVD_WG) X[e] *= scl;
+    __syncthreads();
+    double fro = 0;
+    for (int c = g; c < kx; c += NG) {
+        double* xc = X + int64_t(c) * mx;
+        double s = 0;
+        for (int r = gl; r < mx; r += G) s += xc[r] * xc[r];
+        s = group_sum(s, G);
+        if (s < JACOBI_QUIET)
+            for (int r = gl; r < mx; r += G) xc[r] = 0.0;
+        else if (gl == 0)
+            fro += s;
+    }
+    // squared norm at or below which a column takes part in no rotation
+    const double live = fmax(JACOBI_QUIET, JACOBI_DEAD2 * block_sum<SVD_WG>(fro, wmax));
     __syncthreads();
     const int P = kx + (kx & 1);
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
@@ -1262,11 +1087,9 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                 saa = group_sum(saa, G);
                 sbb = group_sum(sbb, G);
                 sab = group_sum(sab, G);
-                if (fabs(sab) > tol * sqrt(saa * sbb) && sab != 0.0) {
-                    const double zeta = (sbb - saa) / (2.0 * sab);
-                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = cs * t;
+                if (saa > live && sbb > live && sab != 0.0 && fabs(sab) > tol * sqrt(saa) * sqrt(sbb)) {
+                    double cs, sn;
+                    jacobi_rotation(saa, sbb, sab, cs, sn);
                     if (cached) {
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
@@ -1313,13 +1136,17 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
         double s = 0;
         for (int r = gl; r < mx; r += G) s += xc[r] * xc[r];
         s = group_sum(s, G);
+        // a quiet or dead column (squared norm at most the live threshold)
+        // keeps its tiny singular value, marked negative until its direction
+        // is replaced below; an exactly zero column has singular value 0
+        const bool quiet = s <= live;
         const double sc = sqrt(s);
-        const double inv = sc > 0 ? 1.0 / sc : 0.0;
+        const double inv = (quiet || !(sc > 0)) ? 0.0 : 1.0 / sc;
         for (int r = gl; r < mx; r += G) xc[r] *= inv;
-        if (gl == 0) sig[c] = sc;
+        if (gl == 0) sig[c] = quiet ? -ldexp(sc, ex) : ldexp(sc, ex);
     }
     __syncthreads();
-    // exactly rank-deficient blocks: complete the zero columns to an
+    // rank-deficient blocks: complete the zero and quiet columns to an
     // orthonormal set (LAPACK's gesvd returns orthonormal U and V^T too)
     if (g == 0) {
         for (int c = 0; c < kx; c++) {
@@ -1329,7 +1156,7 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                 for (int r = gl; r < mx; r += G) xc[r] = (r == j) ? 1.0 : 0.0;
                 for (int pass = 0; pass < 2; pass++) {
                     for (int d = 0; d < kx; d++) {
-                        if (d == c || (sig[d] == 0.0 && d > c)) continue;
+                        if (d == c || (sig[d] <= 0.0 && d > c)) continue;
                         const double* xd = X + int64_t(d) * mx;
                         double dot = 0;
                         for (int r = gl; r < mx; r += G) dot += xc[r] * xd[r];
@@ -1346,6 +1173,10 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                     break;
                 }
             }
+            // the completed column orthogonalises the later ones
+            if (gl == 0) sig[c] = -sig[c];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     }
     __syncthreads();
@@ -1372,28 +1203,75 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
 
 
 // Register-resident one-sided Jacobi for the large blocks of the batched
-// amplitude engine (rows mx <= 8 * RPT, so columns kx <= 8 * RPT).  32 * RPT
-// threads; pair slot p (4 * RPT slots) is held by 8 lanes of one wave
-// (lane = 8 * row_group + slot % 8),
-// each keeping RPT rows of the slot's two columns in registers.  After every
-// round the columns move along the round-robin ring (slot 0's top fixed,
-// tops shift up, bottoms shift down): by DPP within a wave, through LDS
-// across waves.
+// amplitude engine (rows mx <= 8 * RPT, so columns kx <= 8 * RPT), 32 * RPT
+// threads.  Pair slot p (4 * RPT slots, 8 per wave) is held by 8 lanes of one
+// wave (lane = 8 * row_group + slot % 8), each keeping RPT rows of the slot's
+// top and bottom column in registers.  The 16 NWV columns form 2 NWV
+// half-blocks of 8 (a wave's tops, a wave's bottoms).  A sweep is a
+// round-robin over the half-blocks: in block round 0 every wave pairs the
+// columns inside its two half-blocks (slot p with slot p ^ d, d = 1..7, both
+// lanes of a pair computing the same rotation) and then its tops with its
+// bottoms (8 cross rounds, the bottoms permuted between rounds by DPP xor
+// 1, 3, 1, 7, 1, 3, 1 within each 8-lane group); in block rounds 1 .. 2 NWV - 2
+// the half-blocks move one step along the round-robin ring through LDS and
+// each wave again pairs its tops with its bottoms.  Every pair of columns
+// meets once per sweep, and rounds inside a block round need neither a
+// barrier nor LDS: the waves run them out of step, which hides the latency
+// of each round's reduction and rotation chain behind the other waves.
 // V is not accumulated: after convergence it is recovered as
 // V = M^T U diag(1/sigma) (an LDS-tiled product), which is exact up to
 // eps * sigma_max / sigma_j; columns with sigma_j <= 1e-13 * sigma_max get
 // V = 0 (their products U S V^T are below that level).  The engine only
 // consumes U S and V^T of kept values, so no orthonormal completion is done.
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_QUAD_XOR3 = 0x1B, DPP_ROW_HALF_MIRROR = 0x141;
+
+template<int CTRL>
+__device__ inline int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// value held by the lane whose slot (lane % 8) is this lane's slot ^ D, in
+// the same 8-lane group: xor 1, 2, 3 by quad permutes, xor 7 by the half-row
+// mirror, xor 4, 5, 6 by the mirror followed by a quad permute
+template<int D>
+__device__ inline double xor_slot(double v) {
+    if constexpr (D == 1) return dpp_f64<DPP_QUAD_XOR1>(v);
+    if constexpr (D == 2) return dpp_f64<DPP_QUAD_XOR2>(v);
+    if constexpr (D == 3) return dpp_f64<DPP_QUAD_XOR3>(v);
+    if constexpr (D == 7) return dpp_f64<DPP_ROW_HALF_MIRROR>(v);
+    if constexpr (D == 4) return dpp_f64<DPP_QUAD_XOR3>(dpp_f64<DPP_ROW_HALF_MIRROR>(v));
+    if constexpr (D == 5) return dpp_f64<DPP_QUAD_XOR2>(dpp_f64<DPP_ROW_HALF_MIRROR>(v));
+    if constexpr (D == 6) return dpp_f64<DPP_QUAD_XOR1>(dpp_f64<DPP_ROW_HALF_MIRROR>(v));
+    return v;
+}
+
+template<int D>
+__device__ inline int xor_slot_int(int v) {
+    if constexpr (D == 1) return dpp_i32<DPP_QUAD_XOR1>(v);
+    if constexpr (D == 2) return dpp_i32<DPP_QUAD_XOR2>(v);
+    if constexpr (D == 3) return dpp_i32<DPP_QUAD_XOR3>(v);
+    if constexpr (D == 7) return dpp_i32<DPP_ROW_HALF_MIRROR>(v);
+    if constexpr (D == 4) return dpp_i32<DPP_QUAD_XOR3>(dpp_i32<DPP_ROW_HALF_MIRROR>(v));
+    if constexpr (D == 5) return dpp_i32<DPP_QUAD_XOR2>(dpp_i32<DPP_ROW_HALF_MIRROR>(v));
+    if constexpr (D == 6) return dpp_i32<DPP_QUAD_XOR1>(dpp_i32<DPP_ROW_HALF_MIRROR>(v));
+    return v;
+}
+
+template<int D, int N>
+__device__ inline void move_xor(double (&x)[N], int& id) {
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = xor_slot<D>(x[i]);
+    id = xor_slot_int<D>(id);
+}
 
 template<typename T, int RPT>
-__global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restrict__ tasks) {
-    constexpr int SVR_NT = 32 * RPT, SVR_SLOTS = 4 * RPT, NWV = SVR_NT / WAVE;
-    // exchange buffers per round parity: (top, bottom) x NWV x 8 x RPT doubles
-    // at the wave edges plus slot 0's and slot h-1's tops (2 x 8 x RPT);
-    // reused for the recovery tiles
-    constexpr int SVR_XCH = (2 * NWV + 2) * 8 * RPT;
+__global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restrict__ tasks) {
+    constexpr int NT = 32 * RPT, NWV = NT / WAVE;
+    // block exchange: (NWV + 1) half-blocks of 64 lanes x RPT doubles, i-major
+    // (lanes contiguous); reused for the recovery tiles
     extern __shared__ double xbuf[];
-    __shared__ int ibuf[2][3][NWV];
+    __shared__ int ibuf[NWV + 1][8];
+    __shared__ double rmax[NWV], rfro[NWV];
     __shared__ double smax;
     const SvdDev tk = tasks[blockIdx.x];
     const int m = tk.m, n = tk.n;
@@ -1401,130 +1279,225 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
     const int mx = trans ? n : m, kx = trans ? m : n;
     const T* in = static_cast<const T*>(tk.in);
     const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
-    if (mx > 8 * RPT || kx > 2 * SVR_SLOTS) {
+    if (mx > 8 * RPT || kx > 16 * NWV) {
         // the host routes only fitting blocks here; flag anything else
-        for (int c = tid; c < kx; c += SVR_NT) tk.s[c] = __builtin_nan("");
+        for (int c = tid; c < kx; c += NT) tk.s[c] = __builtin_nan("");
         return;
     }
-    const int gl = lane / 8, pl = lane % 8, p = w * 8 + pl;
-    const int h = (kx + 1) / 2;
-    const bool act = p < h;
-    int ca = p, cb = p + h;
+    const int gl = lane / 8, pl = lane % 8;
+    // wave w starts with the half-blocks of columns 16 w + pl (top, xa) and
+    // 16 w + 8 + pl (bottom, xb); lane 8 gl + pl holds rows gl + 8 i
+    int ca = 16 * w + pl, cb = 16 * w + 8 + pl;
     double xa[RPT], xb[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
         const int r = gl + 8 * i;
-        xa[i] = (act && r < mx && ca < kx) ? double(trans ? in[int64_t(ca) * n + r] : in[int64_t(r) * n + ca]) : 0.0;
-        xb[i] = (act && r < mx && cb < kx) ? double(trans ? in[int64_t(cb) * n + r] : in[int64_t(r) * n + cb]) : 0.0;
+        xa[i] = (r < mx && ca < kx) ? double(trans ? in[int64_t(ca) * n + r] : in[int64_t(r) * n + ca]) : 0.0;
+        xb[i] = (r < mx && cb < kx) ? double(trans ? in[int64_t(cb) * n + r] : in[int64_t(r) * n + cb]) : 0.0;
     }
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
+    // exact power-of-two scaling, quiet and dead columns, as in svd_body
+    int ex;
+    double live;
+    {
+        double amax = 0.0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) amax = fmax(amax, fmax(fabs(xa[i]), fabs(xb[i])));
+        amax = wave_max(amax);
+        if (lane == 0) rmax[w] = amax;
+        __syncthreads();
+        for (int v = 0; v < NWV; v++) amax = fmax(amax, rmax[v]);
+        ex = jacobi_scale_exponent(amax);
+        const double scl = ldexp(1.0, -ex);
+        double sa = 0, sb = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            xa[i] *= scl;
+            xb[i] *= scl;
+            sa += xa[i] * xa[i];
+            sb += xb[i] * xb[i];
+        }
+        sa = sum8(sa);
+        sb = sum8(sb);
+        if (sa < JACOBI_QUIET)
+#pragma unroll
+            for (int i = 0; i < RPT; i++) xa[i] = 0.0;
+        if (sb < JACOBI_QUIET)
+#pragma unroll
+            for (int i = 0; i < RPT; i++) xb[i] = 0.0;
+        // squared Frobenius norm: one lane per slot adds its two columns
+        double fr = gl == 0 ? (sa < JACOBI_QUIET ? 0.0 : sa) + (sb < JACOBI_QUIET ? 0.0 : sb) : 0.0;
+        fr = wave_sum(fr);
+        if (lane == 0) rfro[w] = fr;
+        __syncthreads();
+        fr = 0.0;
+        for (int v = 0; v < NWV; v++) fr += rfro[v];
+        live = fmax(JACOBI_QUIET, JACOBI_DEAD2 * fr);
+    }
     bool converged = kx <= 1;
+    int rot = 0;
+    // rotation of the pair (x of column cx, y of column cy) as seen from the
+    // lane holding x: both lanes of a duplicated pair order it by column
+    // index, so they compute the same rotation and each keeps its own column
+    auto cross_round = [&]() {
+        double saa = 0, sbb = 0, sab = 0, saa1 = 0, sbb1 = 0, sab1 = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i += 2) {
+            saa += xa[i] * xa[i];
+            sbb += xb[i] * xb[i];
+            sab += xa[i] * xb[i];
+            saa1 += xa[i + 1] * xa[i + 1];
+            sbb1 += xb[i + 1] * xb[i + 1];
+            sab1 += xa[i + 1] * xb[i + 1];
+        }
+        saa = sum8(saa + saa1);
+        sbb = sum8(sbb + sbb1);
+        sab = sum8(sab + sab1);
+        if (ca < kx && cb < kx && saa > live && sbb > live && sab != 0.0 && fabs(sab) > tol * sqrt(saa) * sqrt(sbb)) {
+            double cs, sn;
+            jacobi_rotation(saa, sbb, sab, cs, sn);
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const double u = xa[i], v = xb[i];
+                xa[i] = cs * u - sn * v;
+                xb[i] = sn * u + cs * v;
+            }
+            rot = 1;
+        }
+    };
+    // 8 cross rounds: top slot p meets bottom slot p ^ r in round r (the
+    // bottoms move by a DPP lane permutation within each 8-lane group between
+    // rounds: xor 1 and 3 by quad permutes, xor 7 by the half-row mirror), so
+    // each top meets each bottom of the wave once
+    auto cross8 = [&]() {
+        cross_round();
+        move_xor<1>(xb, cb);
+        cross_round();
+        move_xor<3>(xb, cb);
+        cross_round();
+        move_xor<1>(xb, cb);
+        cross_round();
+        move_xor<7>(xb, cb);
+        cross_round();
+        move_xor<1>(xb, cb);
+        cross_round();
+        move_xor<3>(xb, cb);
+        cross_round();
+        move_xor<1>(xb, cb);
+        cross_round();
+    };
+    // intra round: slot p meets slot p ^ D among the tops and among the
+    // bottoms; both slots of a pair hold it (each updates its own column)
+    auto intra = [&](auto dtag) {
+        constexpr int D = decltype(dtag)::value;
+        // the partner's rows are fetched by DPP twice (for the sums, then for
+        // the update) instead of being held: no extra registers
+        const int pa = xor_slot_int<D>(ca), pb = xor_slot_int<D>(cb);
+        double s0 = 0, s1 = 0, s2 = 0, t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const double ya = xor_slot<D>(xa[i]), yb = xor_slot<D>(xb[i]);
+            s0 += xa[i] * xa[i];
+            s1 += ya * ya;
+            s2 += xa[i] * ya;
+            t0 += xb[i] * xb[i];
+            t1 += yb * yb;
+            t2 += xb[i] * yb;
+        }
+        s0 = sum8(s0);
+        s1 = sum8(s1);
+        s2 = sum8(s2);
+        t0 = sum8(t0);
+        t1 = sum8(t1);
+        t2 = sum8(t2);
+        // tops: (lower index, higher index) order fixes the rotation, the
+        // lane keeps its own new column
+        double ca_c = 1.0, ca_s = 0.0, cb_c = 1.0, cb_s = 0.0;
+        bool ra = false, rb = false;
+        {
+            const bool low = ca < pa;
+            const double saa = low ? s0 : s1, sbb = low ? s1 : s0;
+            if (ca < kx && pa < kx && saa > live && sbb > live && s2 != 0.0 && fabs(s2) > tol * sqrt(saa) * sqrt(sbb)) {
+                double cs, sn;
+                jacobi_rotation(saa, sbb, s2, cs, sn);
+                // own' = cs own - sn partner (low) or sn partner + cs own
+                ca_c = cs;
+                ca_s = low ? -sn : sn;
+                ra = true;
+            }
+        }
+        {
+            const bool low = cb < pb;
+            const double saa = low ? t0 : t1, sbb = low ? t1 : t0;
+            if (cb < kx && pb < kx && saa > live && sbb > live && t2 != 0.0 && fabs(t2) > tol * sqrt(saa) * sqrt(sbb)) {
+                double cs, sn;
+                jacobi_rotation(saa, sbb, t2, cs, sn);
+                cb_c = cs;
+                cb_s = low ? -sn : sn;
+                rb = true;
+            }
+        }
+        if (ra || rb) rot = 1;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const double ya = xor_slot<D>(xa[i]), yb = xor_slot<D>(xb[i]);
+            xa[i] = ca_c * xa[i] + ca_s * ya;
+            xb[i] = cb_c * xb[i] + cb_s * yb;
+        }
+    };
     for (int sweep = 0; sweep < tk.max_sweeps && kx > 1; sweep++) {
-        int rot = 0;
-        for (int round = 0; round < 2 * h - 1; round++) {
-            if (act) {
-                // two partial sums per product: halves the FMA dependency chains
-                double saa = 0, sbb = 0, sab = 0, saa1 = 0, sbb1 = 0, sab1 = 0;
+        rot = 0;
+        // block round 0: the pairs inside each half-block, then tops with
+        // bottoms
+        intra(std::integral_constant<int, 1>());
+        intra(std::integral_constant<int, 2>());
+        intra(std::integral_constant<int, 3>());
+        intra(std::integral_constant<int, 4>());
+        intra(std::integral_constant<int, 5>());
+        intra(std::integral_constant<int, 6>());
+        intra(std::integral_constant<int, 7>());
+        cross8();
+        // block rounds 1 .. 2 NWV - 2: round-robin of the 2 NWV half-blocks
+        // (wave 0's top fixed; tops move to the next wave, bottoms to the
+        // previous one, wave 0's bottom becomes wave 1's top and the last
+        // wave's top its own bottom), each followed by the 8 cross rounds
+        for (int b = 1; b < 2 * NWV - 1; b++) {
+            // bottoms out (wave 0's also into the spare slot NWV)
 #pragma unroll
-                for (int i = 0; i < RPT; i += 2) {
-                    saa += xa[i] * xa[i];
-                    sbb += xb[i] * xb[i];
-                    sab += xa[i] * xb[i];
-                    saa1 += xa[i + 1] * xa[i + 1];
-                    sbb1 += xb[i + 1] * xb[i + 1];
-                    sab1 += xa[i + 1] * xb[i + 1];
-                }
-                saa += saa1;
-                sbb += sbb1;
-                sab += sab1;
-                saa = sum8(saa);
-                sbb = sum8(sbb);
-                sab = sum8(sab);
-                if (ca < kx && cb < kx && sab != 0.0 && fabs(sab) > tol * sqrt(saa * sbb)) {
-                    const double zeta = (sbb - saa) / (2.0 * sab);
-                    const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double cs = 1.0 / sqrt(1.0 + t * t);
-                    const double sn = cs * t;
+            for (int i = 0; i < RPT; i++) xbuf[(w * RPT + i) * WAVE + lane] = xb[i];
+            if (gl == 0) ibuf[w][pl] = cb;
+            if (w == 0) {
 #pragma unroll
-                    for (int i = 0; i < RPT; i++) {
-                        const double u = xa[i], v = xb[i];
-                        xa[i] = cs * u - sn * v;
-                        xb[i] = sn * u + cs * v;
-                    }
-                    rot = 1;
-                }
+                for (int i = 0; i < RPT; i++) xbuf[(NWV * RPT + i) * WAVE + lane] = xb[i];
+                if (gl == 0) ibuf[NWV][pl] = cb;
             }
-            if (h < 2) continue;
-            // ring step: top[p] <- top[p-1] (top[1] <- bot[0], top[0] stays),
-            // bot[p] <- bot[p+1] (bot[h-1] <- top[h-1]).  Every lane shifts
-            // both columns by a DPP row shift (slot p -+ 1 of the same wave
-            // is the neighbouring lane of the same 16-lane row); the lanes
-            // whose new column is not that neighbour's (pl = 0 and 7 at the
-            // wave edges, slot 0's fixed top, slot 1's top from slot 0's
-            // bottom, slot h-1's bottom from its own top) were written to LDS
-            // before the barrier and read back in one lane-masked pass each.
-            // The buffers are double-buffered by round parity so one barrier
-            // per round suffices.
-            {
-                const int par = round & 1;
-                double* btop = xbuf + par * SVR_XCH;  // top of slot 8w+7, per wave
-                double* bbot = btop + NWV * 8 * RPT;  // bottom of slot 8w, per wave
-                double* bkeep0 = bbot + NWV * 8 * RPT;  // top of slot 0
-                double* bkeepl = bkeep0 + 8 * RPT;      // top of slot h-1
-                int* itop = ibuf[par][0];
-                int* ibot = ibuf[par][1];
-                const int wl = (h - 1) / 8;  // wave holding slot h-1
-                if (pl == 7) {
+            __syncthreads();
+            if (w + 1 < NWV) {
 #pragma unroll
-                    for (int i = 0; i < RPT; i++) btop[(w * 8 + gl) * RPT + i] = xa[i];
-                    if (gl == 0) itop[w] = ca;
-                }
-                if (pl == 0) {
+                for (int i = 0; i < RPT; i++) xb[i] = xbuf[((w + 1) * RPT + i) * WAVE + lane];
+                cb = ibuf[w + 1][pl];
+            } else {
 #pragma unroll
-                    for (int i = 0; i < RPT; i++) bbot[(w * 8 + gl) * RPT + i] = xb[i];
-                    if (gl == 0) ibot[w] = cb;
-                }
-                if (p == 0) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) bkeep0[gl * RPT + i] = xa[i];
-                    if (gl == 0) ibuf[par][2][0] = ca;
-                }
-                if (p == h - 1) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) bkeepl[gl * RPT + i] = xa[i];
-                    if (gl == 0) ibuf[par][2][1] = ca;
-                }
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < RPT; i++) {
-                    xa[i] = dpp_f64<DPP_ROW_SHR1>(xa[i]);
-                    xb[i] = dpp_f64<DPP_ROW_SHL1>(xb[i]);
-                }
-                ca = __builtin_amdgcn_update_dpp(0, ca, DPP_ROW_SHR1, 0xF, 0xF, true);
-                cb = __builtin_amdgcn_update_dpp(0, cb, DPP_ROW_SHL1, 0xF, 0xF, true);
-                if (pl == 0) {
-                    const double* src = w > 0 ? btop + ((w - 1) * 8 + gl) * RPT : bkeep0 + gl * RPT;
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) xa[i] = src[i];
-                    ca = w > 0 ? itop[w - 1] : ibuf[par][2][0];
-                }
-                if (w == 0 && pl == 1) {
-                    // slot 1's top is slot 0's bottom
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) xa[i] = bbot[gl * RPT + i];
-                    ca = ibot[0];
-                }
-                if (pl == 7 && w + 1 < NWV) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) xb[i] = bbot[((w + 1) * 8 + gl) * RPT + i];
-                    cb = ibot[w + 1];
-                }
-                if (w == wl && p == h - 1) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) xb[i] = bkeepl[gl * RPT + i];
-                    cb = ibuf[par][2][1];
-                }
+                for (int i = 0; i < RPT; i++) xb[i] = xa[i];
+                cb = ca;
             }
+            __syncthreads();
+            // tops out
+#pragma unroll
+            for (int i = 0; i < RPT; i++) xbuf[(w * RPT + i) * WAVE + lane] = xa[i];
+            if (gl == 0) ibuf[w][pl] = ca;
+            __syncthreads();
+            if (w >= 2) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++) xa[i] = xbuf[((w - 1) * RPT + i) * WAVE + lane];
+                ca = ibuf[w - 1][pl];
+            } else if (w == 1) {
+#pragma unroll
+                for (int i = 0; i < RPT; i++) xa[i] = xbuf[(NWV * RPT + i) * WAVE + lane];
+                ca = ibuf[NWV][pl];
+            }
+            __syncthreads();
+            cross8();
         }
         if (!__syncthreads_or(rot)) {
             converged = true;
@@ -1532,7 +1505,7 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
         }
     }
     if (!converged) {
-        for (int c = tid; c < kx; c += SVR_NT) tk.s[c] = __builtin_nan("");
+        for (int c = tid; c < kx; c += NT) tk.s[c] = __builtin_nan("");
         return;
     }
     // sigma, normalised U columns into scratch
@@ -1541,13 +1514,16 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
     double* sig = V + int64_t(kx) * kx;
     if (tid == 0) smax = 0.0;
     __syncthreads();
-    if (act) {
+    {
         double sa = 0, sb = 0;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             sa += xa[i] * xa[i];
             sb += xb[i] * xb[i];
         }
+        // quiet and dead columns keep their tiny singular value (no
+        // orthonormal completion here: the engine reads U S and V^T of kept
+        // values only)
         sa = sqrt(sum8(sa));
         sb = sqrt(sum8(sb));
         const double ia = sa > 0 ? 1.0 / sa : 0.0, ib = sb > 0 ? 1.0 / sb : 0.0;
@@ -1560,8 +1536,8 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
             }
         }
         if (gl == 0) {
-            if (ca < kx) sig[ca] = sa;
-            if (cb < kx) sig[cb] = sb;
+            if (ca < kx) sig[ca] = ldexp(sa, ex);
+            if (cb < kx) sig[cb] = ldexp(sb, ex);
         }
     }
     __syncthreads();
@@ -1579,13 +1555,13 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
     for (int c0 = 0; c0 < kx; c0 += TB) {
         for (int j0 = 0; j0 < kx; j0 += TB) {
             // thread: column c0 + ci, rows j0 + jg + q * JS of the V tile
-            constexpr int JS = SVR_NT / TB, JN = (TB + JS - 1) / JS;
+            constexpr int JS = NT / TB, JN = (TB + JS - 1) / JS;
             const int ci = tid % TB, jg = tid / TB;
             double acc[JN];
 #pragma unroll
             for (int q = 0; q < JN; q++) acc[q] = 0.0;
             for (int r0 = 0; r0 < mx; r0 += KB) {
-                for (int e = tid; e < KB * TB; e += SVR_NT) {
+                for (int e = tid; e < KB * TB; e += NT) {
                     const int rr = e / TB, cc = e % TB, r = r0 + rr;
                     const int c = c0 + cc, j = j0 + cc;
                     As[e] = (r < mx && c < kx) ? double(trans ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]) : 0.0;
@@ -1612,7 +1588,7 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_reg(const SvdDev* __restr
         }
     }
     __syncthreads();
-    const int G = 8, NG = SVR_NT / G;
+    const int G = 8, NG = NT / G;
     svd_write<T>(tk, X, V, sig, tid / G, tid % G, G, NG);
 }
 
@@ -1718,12 +1694,7 @@ void allow_lds(K kernel, int64_t bytes) {
 
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream) {
     if (num <= 0) return;
-    if (variant == 2) {
-        if (dtype == 0)
-            hipLaunchKernelGGL(qr_kernel_chol<double>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
-        else
-            hipLaunchKernelGGL(qr_kernel_chol<float>, dim3(unsigned(num)), dim3(QR_WG), 0, stream, tasks);
-    } else if (variant == 1) {
+    if (variant == 1) {
         if (dtype == 0) {
             allow_lds(qr_kernel_blocked<double>, QB_LDS);
             hipLaunchKernelGGL(qr_kernel_blocked<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
@@ -1752,13 +1723,14 @@ void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, 
 
 namespace {
 template<typename T, int RPT>
-void launch_svd_reg(const SvdDev* tasks, int64_t num, hipStream_t stream) {
-    // exchange buffers (2 parities x (2 x waves + 2) x 8 x RPT), at least the
-    // 32 KiB of the recovery tiles
-    const size_t lds = std::max<size_t>(size_t(2 * (2 * (RPT / 2) + 2) * 8 * RPT), 2 * 32 * 64) * sizeof(double);
-    allow_lds(svd_kernel_reg<T, RPT>, int64_t(lds));
-    hipLaunchKernelGGL((svd_kernel_reg<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
+void launch_svd_blk(const SvdDev* tasks, int64_t num, hipStream_t stream) {
+    // block exchange ((waves + 1) x 64 x RPT), at least the 32 KiB of the
+    // recovery tiles
+    const size_t lds = std::max<size_t>(size_t((RPT / 2 + 1) * 64 * RPT), 2 * 32 * 64) * sizeof(double);
+    allow_lds(svd_kernel_blk<T, RPT>, int64_t(lds));
+    hipLaunchKernelGGL((svd_kernel_blk<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
 }
+
 }  // namespace
 
 void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, int dtype, hipStream_t stream) {
@@ -1766,13 +1738,13 @@ void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, in
     if (rpt > 0) {
         if (rpt != 8 && rpt != 16 && rpt != 24) throw std::runtime_error("svd: unsupported register tile");
         if (dtype == 0) {
-            if (rpt == 8) launch_svd_reg<double, 8>(tasks, num, stream);
-            if (rpt == 16) launch_svd_reg<double, 16>(tasks, num, stream);
-            if (rpt == 24) launch_svd_reg<double, 24>(tasks, num, stream);
+            if (rpt == 8) launch_svd_blk<double, 8>(tasks, num, stream);
+            if (rpt == 16) launch_svd_blk<double, 16>(tasks, num, stream);
+            if (rpt == 24) launch_svd_blk<double, 24>(tasks, num, stream);
         } else {
-            if (rpt == 8) launch_svd_reg<float, 8>(tasks, num, stream);
-            if (rpt == 16) launch_svd_reg<float, 16>(tasks, num, stream);
-            if (rpt == 24) launch_svd_reg<float, 24>(tasks, num, stream);
+            if (rpt == 8) launch_svd_blk<float, 8>(tasks, num, stream);
+            if (rpt == 16) launch_svd_blk<float, 16>(tasks, num, stream);
+            if (rpt == 24) launch_svd_blk<float, 24>(tasks, num, stream);
         }
     } else if (lds_bytes > 0) {
         if (dtype == 0) {

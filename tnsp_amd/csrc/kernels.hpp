@@ -105,27 +105,15 @@ constexpr int SVD_WG = 256;
 constexpr int64_t LDS_CLASS[3] = {48 * 1024, 76 * 1024, 152 * 1024};
 constexpr int64_t QR_LDS_MAX = LDS_CLASS[2];
 constexpr int64_t SVD_LDS_MAX = LDS_CLASS[2];
-// CholeskyQR2 kernel (variant 2 of launch_qr): QR (not LQ) blocks with
-// m >= n, at most 16 * QC_T columns, beyond the LDS-resident kernel; its
-// scratch holds Q1 (m n) plus the R1 / R2 tiles and the inverses of the
-// diagonal tiles
-constexpr int QC_T = 12;
-constexpr int64_t QC_EXTRA = int64_t(2 * QC_T * QC_T + QC_T) * 256;
-inline bool qr_uses_chol(int64_t m, int64_t n, int lq) {
-    return !lq && m >= n && n <= 16 * QC_T && (m * n + (m < n ? m : n)) * 8 > LDS_CLASS[2];
-}
-inline int64_t qr_doubles(int64_t m, int64_t n, int lq) {
-    const int64_t k = m < n ? m : n;
-    return m * n + (qr_uses_chol(m, n, lq) ? QC_EXTRA : k);
-}
+// scratch of a QR / LQ task beyond LDS: the block and its taus
+inline int64_t qr_doubles(int64_t m, int64_t n) { return m * n + (m < n ? m : n); }
 inline int64_t svd_doubles(int64_t m, int64_t n) {
     const int64_t mx = m > n ? m : n, kx = m < n ? m : n;
     return mx * kx + kx * kx + kx;
 }
 // variant 1: blocked Householder with the panel in LDS (global scratch
-// blocks with at most QB_MAX_ROWS rows); variant 2: CholeskyQR2
-// (qr_uses_chol blocks); otherwise LDS (lds_bytes > 0) or unblocked global
-// scratch
+// blocks with at most QB_MAX_ROWS rows); otherwise LDS (lds_bytes > 0) or
+// unblocked global scratch
 void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, int dtype, hipStream_t stream);
 constexpr int64_t QB_LDS = 156 * 1024;
 // rows for which the blocked kernel's panel keeps at least 4 columns

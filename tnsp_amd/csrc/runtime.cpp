@@ -5,7 +5,6 @@
 #include <atomic>
 #include <cstring>
 #include <stdexcept>
-#include <type_traits>
 
 namespace tns {
 
@@ -319,7 +318,7 @@ void Batch::qr(const std::vector<QrTask>& ts, void* const* bufs, int64_t es, dou
         q.m = t.m;
         q.n = t.n;
         q.lq = t.lq;
-        cur += qr_doubles(t.m, t.n, t.lq);
+        cur += qr_doubles(t.m, t.n);
         const double M = double(std::max(t.m, t.n)), N = double(std::min(t.m, t.n));
         const double fl = 4.0 * M * N * N - 4.0 * N * N * N / 3.0;
         pend_units_[QR] += fl;
@@ -340,22 +339,18 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
     // part 0: global scratch, parts 1..3: LDS classes (largest first),
     // parts 4..6: register-resident SVD with 24 / 16 / 8 rows per lane,
-    // part 7: blocked QR, part 8: CholeskyQR2
-    constexpr int NPART = 9;
+    // part 7: blocked QR
+    constexpr int NPART = 8;
     std::vector<D> parts[NPART];
     double units[NPART] = {};
     int64_t need[NPART] = {};
-    static const int part_rpt[NPART] = {0, 0, 0, 0, 24, 16, 8, 1, 2};
+    static const int part_rpt[NPART] = {0, 0, 0, 0, 24, 16, 8, 1};
     for (size_t i : order) {
         const int64_t bytes = doubles(tasks[i].m, tasks[i].n) * 8;
         const int64_t rows = std::max<int64_t>(tasks[i].m, tasks[i].n);
         int p = 0;
-        bool chol = false;
-        if constexpr (std::is_same<D, QrDev>::value) chol = qr_uses_chol(tasks[i].m, tasks[i].n, tasks[i].lq);
         if (kind == SVD && fast_svd_ && bytes > LDS_CLASS[1] && rows <= SVD_REG_MAX_ROWS) {
             p = rows <= 64 ? 6 : (rows <= 128 ? 5 : 4);
-        } else if (chol) {
-            p = 8;
         } else if (kind == QR && bytes > lds_max && rows <= QB_MAX_ROWS) {
             p = 7;
         } else if (bytes <= lds_max) {
@@ -383,12 +378,7 @@ void Batch::flush_decomp(Kind kind, std::vector<D>& tasks, std::vector<double>& 
     cost.clear();
 }
 
-namespace {
-// working set of the LDS-resident QR kernel (the block and its taus)
-int64_t qr_lds_doubles(int64_t m, int64_t n) { return m * n + std::min(m, n); }
-}  // namespace
-
-void Batch::flush_qr() { flush_decomp(QR, pqr_, pqr_cost_, QR_LDS_MAX, qr_lds_doubles); }
+void Batch::flush_qr() { flush_decomp(QR, pqr_, pqr_cost_, QR_LDS_MAX, qr_doubles); }
 
 void Batch::svd(const std::vector<SvdTask>& ts, void* const* bufs, int64_t es, double* scratch) {
     double* cur = scratch;
@@ -481,7 +471,6 @@ void Batch::run(hipStream_t st) {
             check_hip(hipEventRecord(eb, st), "timer record");
             int k = int(c.kind);
             if (c.kind == QR && c.rpt == 1) k = KernelTimer::QR_BLOCKED;
-            if (c.kind == QR && c.rpt == 2) k = KernelTimer::QR_CHOL;
             if (c.kind == SVD && c.rpt == 8) k = KernelTimer::SVD_REG8;
             if (c.kind == SVD && c.rpt == 16) k = KernelTimer::SVD_REG16;
             if (c.kind == SVD && c.rpt == 24) k = KernelTimer::SVD_REG24;

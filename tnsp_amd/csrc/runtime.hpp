@@ -60,7 +60,7 @@ class Staging {
 struct KernelTimer {
     // one entry per kernel (variant) so that launch averages compare with a
     // kernel trace: QR / SVD are the LDS and unblocked-global kernels
-    enum { PERM, FILL, GEMM, QR, SVD, DIAG, QR_BLOCKED, SVD_REG8, SVD_REG16, SVD_REG24, QR_CHOL, NKIND };
+    enum { PERM, FILL, GEMM, QR, SVD, DIAG, QR_BLOCKED, SVD_REG8, SVD_REG16, SVD_REG24, NKIND };
     struct Acc {
         double ms = 0, units = 0, launches = 0;
     } acc[NKIND];

@@ -92,8 +92,7 @@ class AmplitudeEngine:
         _lib.check(_lib.lib().tns_engine_stats(self.handle, vals, len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, list(vals)))
 
-    KERNEL_KINDS = ["permute", "fill", "gemm", "qr", "svd", "diag", "qr_blocked", "svd_reg8", "svd_reg16", "svd_reg24",
-                    "qr_chol"]
+    KERNEL_KINDS = ["permute", "fill", "gemm", "qr", "svd", "diag", "qr_blocked", "svd_reg8", "svd_reg16", "svd_reg24"]
 
     def set_profile(self, on=True):
         _lib.check(_lib.lib().tns_engine_set_profile(self.handle, int(bool(on))))
