@@ -192,38 +192,51 @@ def replace(sites, config, cut, positions, occupations):
     raise ValueError("replace of this site pair is outside the sweep's bonds")
 
 
-def sweep(sites, config, cut, uniforms, alpha=1.0, t=1.0, V=1.0):
-    """One ``SweepSampling.__call__`` of the tV model (sampling.py:145-154):
-    bonds in ``bond_order``; on a bond with one particle the hop target, on a
-    doubly occupied bond (only the diagonal element when V != 0) the
-    configuration itself, whose ``replace`` is ``hole(())``; an empty bond has
-    no element and draws nothing.  ``uniforms[k]`` is the acceptance draw of
-    bond k.  Returns the final configuration, the per-bond trace of
-    (|ws| before, |wss| or None, accepted) and the final |ws|."""
+def sweep(sites, config, cut, uniforms, alpha=1.0, t=1.0, V=1.0, table=None, target_uniforms=None):
+    """One ``SweepSampling.__call__`` (sampling.py:117-154): bonds in
+    ``bond_order``; the bond's occupations select their targets in the
+    element table (``table``: {in occupations: {out occupations: element}},
+    the keys of ``sampling.element_pool``; default the tV model's: a hop
+    between a filled and an empty site when t != 0, the diagonal of a doubly
+    occupied bond when V != 0); occupations without elements draw nothing.
+    One target is drawn uniformly (``target_uniforms[k]`` picks it when a key
+    has several: index floor(u n)); a target equal to the configuration is
+    ``hole(())``, another one ``replace``; the acceptance is
+    |wss/ws|^(2 alpha) n / n' against ``uniforms[k]``.  Returns the final
+    configuration, the per-bond trace of (|ws| before, |wss| or None,
+    accepted) and the final |ws|."""
     from .sampling import bond_order
 
+    if table is None:
+        table = {}
+        if t != 0:
+            table[(0, 1)] = {(1, 0): None}
+            table[(1, 0)] = {(0, 1): None}
+        if V != 0:
+            table[(1, 1)] = {(1, 1): None}
     cfg = np.array(config, dtype=np.int64).copy()
     env = Environments(project(sites, cfg), cut)
     ws = value(env.hole())
     trace = []
     for k, (a, b) in enumerate(bond_order(len(cfg), len(cfg[0]))):
-        na, nb = int(cfg[a]), int(cfg[b])
-        if na == 0 and nb == 0 or (na == 1 and nb == 1 and V == 0):
+        key = (int(cfg[a]), int(cfg[b]))
+        if key not in table or not table[key]:
             trace.append((abs(ws), None, False))
             continue
-        if na == nb:
+        outs = list(table[key])
+        n = len(outs)
+        target = outs[0] if n == 1 else outs[min(int(target_uniforms[k] * n), n - 1)]
+        ns = len(table[target])
+        if target == key:
             wss = value(Environments(project(sites, cfg), cut).hole())
-            new = None
         else:
-            new = (nb, na)
-            wss = value(replace(sites, cfg, cut, (a, b), new))
-        p = abs(wss / ws) ** (2 * alpha)  # one target before and after: n / n' = 1
+            wss = value(replace(sites, cfg, cut, (a, b), target))
+        p = abs(wss / ws) ** (2 * alpha) * n / ns
         acc = bool(uniforms[k] < p)
         trace.append((abs(ws), abs(wss), acc))
         if acc:
             ws = wss
-            if new is not None:
-                cfg[a], cfg[b] = new
+            cfg[a], cfg[b] = target
     return cfg, trace, abs(ws)
 
 

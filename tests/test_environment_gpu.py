@@ -14,7 +14,7 @@ from test_sampler import small_lattice
 pytestmark = pytest.mark.gpu
 
 
-def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1, lanes=1):
+def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1, lanes=1, speculate=False, table=None):
     from oracle import environment as E
     from tnsp_amd import peps as pp
     from tnsp_amd.engine import AmplitudeEngine
@@ -23,16 +23,24 @@ def _run(L1, L2, N, cut, chains, seed, diagonal, sweeps=1, lanes=1):
     spec, storages, sites = small_lattice(L1, L2, N, seed=seed)
     eng = AmplitudeEngine(pp.Lattice(spec, storages), chunk=64, lanes=1)
     start = np.stack(configs(L1, L2, N, chains, seed=seed + 1))
-    smp = EnvironmentSampler(eng, start, seed=seed + 2, cut=cut, diagonal=diagonal, lanes=lanes)
+    smp = EnvironmentSampler(eng, start, seed=seed + 2, cut=cut, diagonal=diagonal, lanes=lanes, speculate=speculate,
+                             hamiltonian=table)
     gen = np.random.Generator(np.random.PCG64(seed + 2))
+    # target draws: the sampler's second generator, seeded with (seed, 1)
+    gen_t = np.random.Generator(np.random.PCG64(np.random.SeedSequence((seed + 2, 1))))
     nb = len(smp.bonds)
     cur = start.astype(np.int64)
     for _ in range(sweeps):
         smp.sweep()
         u = np.stack([gen.random(chains) for _ in range(nb)])  # (bonds, chains), the sampler's draw order
+        tu = gen_t.random((nb, chains)) if table is not None and smp.moves.multi else None
         want, last = [], []
         for b in range(chains):
-            cfg, trace, ws = E.sweep(sites, cur[b], cut, u[:, b], V=1.0 if diagonal == "hole" else 0.0)
+            if table is not None:
+                cfg, trace, ws = E.sweep(sites, cur[b], cut, u[:, b], table=table,
+                                         target_uniforms=None if tu is None else tu[:, b])
+            else:
+                cfg, trace, ws = E.sweep(sites, cur[b], cut, u[:, b], V=1.0 if diagonal == "hole" else 0.0)
             want.append(cfg)
             last.append(ws)
         cur = np.stack(want)
@@ -46,6 +54,55 @@ def test_environment_sweep_matches_oracle_truncated():
     refresh psi through hole(())."""
     smp = _run(4, 4, 8, 4, chains=3, seed=20, diagonal="hole")
     assert smp.accepted > 0 and smp.refreshed > 0
+
+
+def test_environment_sweep_speculative_matches_oracle():
+    """The same with the sweep run speculatively and replayed."""
+    smp = _run(4, 4, 8, 4, chains=3, seed=20, diagonal="hole", speculate=True)
+    assert smp.accepted > 0 and smp.refreshed > 0
+
+
+def test_environment_sweep_speculation_sweeps_again():
+    """Six chains at boundary cut 4: hole(()) and the carried psi differ
+    enough that one chain's speculative decisions disagree with the replay
+    (the oracle shows chain trajectories that differ between the two
+    rules); that chain is swept again step by step, and every chain still
+    takes the oracle's decisions and ends with its psi."""
+    smp = _run(4, 4, 8, 4, chains=6, seed=20, diagonal="hole", speculate=True)
+    assert smp.respun > 0 and smp.refreshed > 0
+
+
+def test_environment_sweep_element_table_with_several_targets():
+    """A bond operator whose table has two targets for (0, 1) (tV plus
+    mu n_1: the hop and the diagonal) and one for (1, 0): targets drawn from
+    the second uniform, acceptance with n / n' = 2 and 1/2, diagonal targets
+    through hole(()); the sampler takes the oracle's decisions at a
+    truncating cut, speculative and step by step."""
+    from oracle import sampling
+    from test_elements import _mu_operator
+
+    table = {k: {o: float(v.data[0]) for o, v in d.items()} for k, d in
+             sampling.element_pool(_mu_operator(1.0, 0.7, 0.5)).items()}
+    smp = _run(4, 4, 8, 4, chains=4, seed=21, diagonal="hole", table=table)
+    assert smp.accepted > 0 and smp.refreshed > 0
+    smp = _run(4, 4, 8, 4, chains=4, seed=21, diagonal="hole", table=table, speculate=True)
+    assert smp.accepted > 0
+
+
+def test_tensor_element_of_a_gpu_operator():
+    """tensor_element of a drop-in Tensor equals the table of the same
+    operator's host blocks (and the oracle's pool)."""
+    from oracle import sampling
+    from helpers import to_gpu
+    from test_elements import _host_table, _mu_operator
+    from tnsp_amd.elements import tensor_element
+
+    H = _mu_operator(1.0, 0.7, 0.5)
+    got = tensor_element(to_gpu(H))
+    want = _host_table(H)
+    assert list(got) == list(want)
+    assert all(list(got[k].items()) == list(want[k].items()) for k in want)
+    assert list(got) == list(sampling.element_pool(H))
 
 
 def test_environment_sweep_lanes_match_oracle():

@@ -31,6 +31,25 @@ amplitude computed afresh, and the draw compares it with the carried psi.
 ``diagonal="keep"`` skips it (the two amplitudes differ only by the
 boundary-MPS truncation, so the step accepts with probability one up to
 that difference).
+
+Step by step, each lane refreshes hole(()) with an amplitude engine of its
+own, so concurrent lanes do not queue on one engine handle.
+
+A hole(()) changes only the carried psi, never the configuration, so no
+environment depends on it: only later acceptance tests read it.  With
+``speculate=True`` the ``"hole"`` sweep runs speculatively: the lockstep pass
+decides with the carried psi and records every hop (its replace value and
+decision) and every doubly occupied bond (a hole(()) request whenever the
+chain moved since its last one); all requests of the sweep are then
+evaluated by the engine in one batch, and the decisions are replayed in bond
+order with the true psi.  A chain whose replayed decisions all agree
+followed tetragono's trajectory exactly and takes the replayed psi; a chain
+with any disagreement is swept again from its start on the step-by-step path
+with the same uniforms.  Either way every chain ends where the step-by-step
+sweep ends, with the same psi.  This pays only where hole(()) and the
+carried psi nearly agree (a mild truncation): on the random 8x8 D=8 PEPS at
+Dc=64 they differ by a median factor of e^1.7 and nearly every chain is swept
+again, so the default is step by step.
 """
 import threading
 from concurrent.futures import ThreadPoolExecutor
@@ -38,8 +57,64 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from . import batch as Bt
+from .elements import Moves, tensor_element, tv_table
 from .peps import _one
 from .sampler import bond_order
+
+
+class _Speculation:
+    """Record of one lane's speculative ``"hole"`` sweep: per bond the hop
+    chains with their replace values and decisions, the doubly occupied
+    chains with the hole(()) request each one reads; request r of chain c is
+    the configuration c had when it reached a doubly occupied bond after
+    moving (requests 0 .. n-1: the sweep-start psi of the n chains, which is
+    hole(()) itself)."""
+
+    def __init__(self, psi0):
+        n = len(psi0)
+        self.psi0 = psi0.copy()
+        self.req = np.arange(n)  # current request of each chain, -1 after a move
+        self.owner = [np.arange(n)]
+        self.cfgs = []
+        self.nreq = n
+        self.events = []  # (bond, kind, chains, values, decisions)
+
+    def hop(self, k, idx, wss, acc, n, ns):
+        self.events.append((k, 0, idx, wss, acc, n, ns))
+        self.req[idx[acc]] = -1
+
+    def diag(self, k, idx, configs, n, ns):
+        new = idx[self.req[idx] < 0]
+        if len(new):
+            self.req[new] = self.nreq + np.arange(len(new))
+            self.nreq += len(new)
+            self.owner.append(new)
+            self.cfgs.append(configs[new].copy())
+        self.events.append((k, 1, idx, self.req[idx].copy(), None, n, ns))
+
+    def replay(self, u, alpha, hole):
+        """Decisions in bond order with the true psi (``hole``: |hole(())| of
+        every request).  Returns the final psi, the chains whose replayed
+        decisions differ from the recorded ones anywhere, and, counted over
+        the others, proposals, acceptances and engine refreshes."""
+        psi = self.psi0.copy()
+        bad = np.zeros(len(psi), dtype=bool)
+        acc_n = np.zeros(len(psi), dtype=np.int64)
+        prop_n = np.zeros(len(psi), dtype=np.int64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for k, kind, idx, val, acc, n, ns in self.events:
+                wss = val if kind == 0 else hole[val]
+                # the expression of the step-by-step path, elementwise
+                t = u[k, idx] < (wss / psi[idx]) ** (2 * alpha) * n / ns
+                if kind == 0:
+                    bad[idx[t != acc]] = True
+                    prop_n[idx] += 1
+                    acc_n[idx] += t
+                psi[idx[t]] = wss[t]
+        owner = np.concatenate(self.owner)
+        good = ~bad
+        refreshed = int(good[owner[len(psi):]].sum())
+        return psi, bad, int(prop_n[good].sum()), int(acc_n[good].sum()), refreshed
 
 
 class EnvironmentSampler:
@@ -49,11 +124,22 @@ class EnvironmentSampler:
     ``sweep()`` is ``SweepSampling.__call__``; ``local_energy()`` the
     observer's energy and ``holes()`` ``Configuration.holes``, all over the
     same cached environments.  ``cut`` overrides the PEPS's boundary-MPS
-    bond dimension; ``diagonal`` chooses how doubly occupied bonds are
-    treated (see the module notes); ``lanes`` splits the chains into groups
-    that run on their own host threads and HIP streams."""
+    bond dimension; ``diagonal`` chooses how diagonal targets (a doubly
+    occupied bond of the tV model) are treated and ``speculate`` how
+    ``"hole"`` is executed (see the module notes; ``respun`` counts chains
+    swept again after a disagreement); ``lanes`` splits the chains into
+    groups that run on their own host threads and HIP streams.
 
-    def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole", lanes=1):
+    ``hamiltonian``: the bond operator whose element table
+    (``elements.tensor_element`` of a drop-in ``Tensor`` with edges I0, I1,
+    O0, O1, or the table itself as a dict) drives the sweep; by default the
+    tV model's (t, V).  A table with several targets per key draws the target
+    from a second uniform per chain and bond, from a generator seeded with
+    (seed, 1).  The observer's energy and the gradient use the tV elements
+    and refuse another operator."""
+
+    def __init__(self, engine, configs, t=1.0, V=1.0, alpha=1.0, seed=0, cut=None, diagonal="hole", lanes=1,
+                 speculate=False, hamiltonian=None):
         if diagonal not in ("hole", "keep"):
             raise ValueError('diagonal must be "hole" or "keep"')
         self.lanes = max(1, int(lanes))
@@ -64,10 +150,23 @@ class EnvironmentSampler:
         self.cut = spec.cut if cut is None else cut
         self.t, self.V, self.alpha = float(t), float(V), float(alpha)
         self.diagonal = diagonal
+        self.speculate = bool(speculate)
+        self._spec = None
+        self.respun = 0
         self.configs = np.ascontiguousarray(np.asarray(configs), dtype=np.int8).copy()
         if self.configs.ndim != 3 or self.configs.shape[1:] != (self.L1, self.L2):
             raise ValueError("configs must have shape (chains, L1, L2)")
         self.gen = np.random.Generator(np.random.PCG64(seed))
+        self.gen_target = np.random.Generator(np.random.PCG64(np.random.SeedSequence((seed, 1))))
+        if hamiltonian is None:
+            table = tv_table(self.t, self.V)
+        elif isinstance(hamiltonian, dict):
+            table = hamiltonian
+        else:
+            table = tensor_element(hamiltonian)
+        self.custom_operator = hamiltonian is not None
+        self.moves = Moves(table)
+        self._v = None
         self.bonds = bond_order(self.L1, self.L2)
         self._bond_index = {bd: k for k, bd in enumerate(self.bonds)}
         self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
@@ -90,6 +189,10 @@ class EnvironmentSampler:
         self._u = None
         self._pool = None
         self._streams = []
+        # hole(()) engines of lanes 1, 2, ... (lane 0 uses ``engine``): the
+        # step-by-step refreshes of concurrent lanes do not queue on one handle
+        self._lane = 0
+        self._lane_engines = {}
         from .sampler import _fermion_order_index, hole_name_order
 
         self._hole_pos = {n: i for i, n in enumerate(hole_name_order(self.L1, self.L2))}
@@ -102,6 +205,8 @@ class EnvironmentSampler:
         again here and in the engine; chains keep their configurations and
         psi is recomputed."""
         self.engine.refresh_sites()
+        for eng in self._lane_engines.values():
+            eng.refresh_sites()
         self._proj = [[(self.lattice.projected(l1, l2, 0), self.lattice.projected(l1, l2, 1))
                        for l2 in range(self.L2)] for l1 in range(self.L1)]
         self.psi = np.abs(self.engine.evaluate(self.configs, cut=self.cut))
@@ -139,17 +244,23 @@ class EnvironmentSampler:
 
     # -- one bond ---------------------------------------------------------------------
     def _term(self, a, b, replace):
-        """``_single_term`` on bond (a, b) for every chain: one uniform per
-        chain; chains with one particle on the bond propose the hop and read
-        psi(s') from ``replace(idx, t0, t1)``."""
+        """``_single_term`` on bond (a, b) for every chain: the bond's
+        occupations select their targets in the element table, one target is
+        drawn per chain and accepted with |psi(s')/psi(s)|^(2 alpha) n / n'
+        against one uniform per chain.  An off-diagonal target reads psi(s')
+        from ``replace(idx, t0, t1)``; a target equal to the configuration
+        reads hole(()) (module notes)."""
         na = self.configs[:, a[0], a[1]].copy()
         nb = self.configs[:, b[0], b[1]].copy()
-        u = self._u[self._bond_index[(a, b)]]
-        hop = np.nonzero(na != nb)[0]
-        dbl = np.nonzero((na == 1) & (nb == 1))[0] if self.V != 0 else np.zeros(0, dtype=np.int64)
+        k = self._bond_index[(a, b)]
+        u = self._u[k]
+        active, ta, tb, n, ns = self.moves.draw(na, nb, None if self._v is None else self._v[k])
+        moved = (ta != na) | (tb != nb)
+        hop = np.nonzero(active & moved)[0]
+        dbl = np.nonzero(active & ~moved)[0]
         if len(hop):
-            t0 = self._site(a[0], a[1], nb[hop])
-            t1 = self._site(b[0], b[1], na[hop])
+            t0 = self._site(a[0], a[1], ta[hop])
+            t1 = self._site(b[0], b[1], tb[hop])
             wss = np.abs(replace(hop, t0, t1).values())
             if self.recorder is not None:
                 with self._lock:
@@ -158,25 +269,47 @@ class EnvironmentSampler:
             self.proposals += len(hop)
             with np.errstate(divide="ignore", invalid="ignore"):
                 # a carried psi of zero (every sector cut away) accepts any
-                # non-zero proposal, as the tensor ratio of the reference does
-                acc = u[hop] < (wss / self.psi[hop]) ** (2 * self.alpha)
+                # non-zero proposal, as the tensor ratio of the reference does;
+                # the factor n / n' in tetragono's order of operations
+                acc = u[hop] < (wss / self.psi[hop]) ** (2 * self.alpha) * n[hop] / ns[hop]
             take = hop[acc]
-            self.configs[take, a[0], a[1]], self.configs[take, b[0], b[1]] = nb[take], na[take]
+            if self._spec is not None:
+                self._spec.hop(k, hop, wss, acc, n[hop], ns[hop])
+            self.configs[take, a[0], a[1]], self.configs[take, b[0], b[1]] = ta[take], tb[take]
             self.psi[take] = wss[acc]
             self.accepted += int(acc.sum())
             self._hole_cache[take] = np.nan
-        if len(dbl) and self.diagonal == "hole":
+        if len(dbl) and self.diagonal == "hole" and self._spec is not None:
+            self._spec.diag(k, dbl, self.configs, n[dbl], ns[dbl])
+        elif len(dbl) and self.diagonal == "hole":
             # hole(()) is recomputed only for chains that moved since it was
             # last known, as the lazy graph's cache does
             need = dbl[np.isnan(self._hole_cache[dbl])]
             if len(need):
-                with self._lock:  # one engine handle: one caller at a time
-                    self._hole_cache[need] = np.abs(self.engine.evaluate(self.configs[need], cut=self.cut))
+                self._hole_cache[need] = np.abs(self._engine_of_lane().evaluate(self.configs[need], cut=self.cut))
                 self.refreshed += len(need)
             wss = self._hole_cache[dbl]
             with np.errstate(divide="ignore", invalid="ignore"):
-                acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha)
+                acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha) * n[dbl] / ns[dbl]
             self.psi[dbl[acc]] = wss[acc]
+
+    def _engine_of_lane(self):
+        """The amplitude engine this lane evaluates hole(()) with: the
+        sampler's own for lane 0, one of the same PEPS and precision per
+        further lane (made on first use), so that lanes refresh
+        concurrently on their own streams."""
+        if self._lane == 0:
+            return self.engine
+        with self._lock:
+            eng = self._lane_engines.get(self._lane)
+            if eng is None:
+                from .engine import AmplitudeEngine
+
+                # a few configurations per call: planned on this thread and
+                # one helper, not on a pool of its own per lane
+                eng = AmplitudeEngine(self.lattice, threads=2, chunk=256, lanes=1, precision=self.engine.precision)
+                self._lane_engines[self._lane] = eng
+        return eng
 
     # -- passes -----------------------------------------------------------------------
     # -- row environments (single_layer_auxiliaries.py:254-312, 437-442) --------------
@@ -232,6 +365,8 @@ class EnvironmentSampler:
                 else:
                     self.psi = np.abs(rt[0].values())
                     self._hole_cache = self.psi.copy()
+                    if self._spec is not None:
+                        self._spec = _Speculation(self.psi)
             lt = self._left_start(one, up)
             for l2 in range(L2 - 1):
                 def replace(idx, t0, t1, lt=lt, l2=l2):
@@ -350,6 +485,9 @@ class EnvironmentSampler:
         from ``replace`` over the cached environments, the diagonal term
         V n_i n_j.  Chains whose psi(s) is exactly zero get NaN (the observer
         skips such configurations).  ``configs`` defaults to the chains."""
+        if self.custom_operator:
+            raise NotImplementedError("the observer's energy is the tV model's; this sampler was given another "
+                                      "bond operator")
         cfg = self.configs if configs is None else np.ascontiguousarray(np.asarray(configs), dtype=np.int8)
         out = np.zeros(len(cfg))
 
@@ -377,18 +515,86 @@ class EnvironmentSampler:
         One uniform per chain and bond, drawn for all chains bond by bond,
         whatever the number of lanes."""
         self._u = self.gen.random((len(self.bonds), len(self.configs)))
+        self._v = self.gen_target.random((len(self.bonds), len(self.configs))) if self.moves.multi else None
+        diagonal_targets = any(k in outs for k, outs in self.moves.targets.items())
+        if self.diagonal == "hole" and self.speculate and self.recorder is None and diagonal_targets:
+            return self._sweep_speculative(log)
 
         def run(child, idx):
             child.configs = self.configs[idx].copy()
             child.psi = self.psi[idx].copy()
             child._hole_cache = np.full(len(idx), np.nan)
             child._u = self._u[:, idx]
+            child._v = None if self._v is None else self._v[:, idx]
+            child._spec = None
             child._horizontal(log)
             child._vertical(log)
             self.configs[idx] = child.configs
             self.psi[idx] = child.psi
 
         self._in_lanes(len(self.configs), run)
+        return self.configs
+
+    def _sweep_speculative(self, log=None):
+        """The ``"hole"`` sweep run speculatively, then replayed (module notes)."""
+        start = self.configs.copy()
+        records = []
+
+        def run(child, idx):
+            child.configs = self.configs[idx].copy()
+            child.psi = self.psi[idx].copy()
+            child._hole_cache = np.full(len(idx), np.nan)
+            child._u = self._u[:, idx]
+            child._v = None if self._v is None else self._v[:, idx]
+            child._spec = True  # replaced by the record once row 0 gives psi
+            child._horizontal(log)
+            child._vertical(log)
+            records.append((idx, child._spec, child.configs))
+
+        before = (self.proposals, self.replaced, self.accepted)
+        self._in_lanes(len(self.configs), run)
+        # the lanes counted speculative decisions: the replay counts again
+        self.proposals, self.replaced, self.accepted = before
+        cfgs = [c for _, rec, _ in records for c in rec.cfgs]
+        if cfgs:
+            with self._lock:
+                fresh = np.abs(self.engine.evaluate(np.concatenate(cfgs), cut=self.cut))
+        if log:
+            log(f"hole(()) of {sum(len(c) for c in cfgs)} configurations evaluated")
+        off = 0
+        redo = []
+        for idx, rec, cfg in records:
+            nfresh = rec.nreq - len(idx)
+            hole = np.concatenate([rec.psi0, fresh[off:off + nfresh]]) if nfresh else rec.psi0
+            off += nfresh
+            psi, bad, props, accs, refreshed = rec.replay(self._u[:, idx], self.alpha, hole)
+            self.configs[idx] = cfg
+            self.psi[idx] = psi
+            self.proposals += props
+            self.replaced += props
+            self.accepted += accs
+            self.refreshed += refreshed
+            redo.append(idx[bad])
+        redo = np.concatenate(redo)
+        if len(redo):
+            # a disagreement: these chains take the step-by-step sweep again
+            # from their start, on the same uniforms
+            self.respun += len(redo)
+            child = self._child()
+            child.configs = start[redo].copy()
+            child.psi = self.psi[redo].copy()
+            child._hole_cache = np.full(len(redo), np.nan)
+            child._u = self._u[:, redo]
+            child._v = None if self._v is None else self._v[:, redo]
+            child._spec = None
+            child._horizontal(log)
+            child._vertical(log)
+            self.configs[redo] = child.configs
+            self.psi[redo] = child.psi
+            self.proposals += child.proposals
+            self.replaced += child.replaced
+            self.accepted += child.accepted
+            self.refreshed += child.refreshed
         return self.configs
 
     # -- holes (sampling_lattice/lattice.py:362-416) ------------------------------------
@@ -564,6 +770,8 @@ class EnvironmentSampler:
         one group runs on the caller's thread and stream."""
         groups = [g for g in np.array_split(np.arange(n), min(self.lanes, max(1, n))) if len(g)]
         children = [self._child() for _ in groups]
+        for i, c in enumerate(children):
+            c._lane = i
         if len(groups) == 1:
             run(children[0], groups[0])
         else:

@@ -157,6 +157,42 @@ class Lattice:
             self.sites.append(row)
         self._projected = {}
 
+    @classmethod
+    def from_saved(cls, saved, cut, dtype=None):
+        """The lattice of a saved tetragono ``SamplingLattice``
+        (``latticeio.read_lattice``) on the GPU: its site tensors as they
+        are, one physical orbit per site (edge ``P0``), a FermiU1 or FermiZ2
+        symmetry.  ``cut`` is the boundary-MPS bond dimension (tetragono
+        keeps it outside the lattice); the particle number comes from the
+        ``T`` edge of site (0, 0).  ``spec.segments`` is empty: the bonds are
+        whatever the file holds."""
+        if saved.kind not in ("FermiU1", "FermiZ2"):
+            raise ValueError(f"the engine takes FermiU1 or FermiZ2 lattices, not {saved.kind}")
+        s00 = saved.site(0, 0)
+        if "T" not in s00.names:
+            raise ValueError("site (0, 0) has no T edge")
+        for l1 in range(saved.L1):
+            for l2 in range(saved.L2):
+                names = saved.site(l1, l2).names
+                if "P0" not in names or any(n.startswith("P") and n != "P0" for n in names):
+                    raise ValueError(f"site ({l1}, {l2}) must have exactly one physical edge P0, has {names}")
+        q = s00.edges[s00.names.index("T")].segments[0][0][0]
+        N = -q if saved.kind == "FermiU1" else q
+        self = cls.__new__(cls)
+        self.spec = LatticeSpec(saved.kind, saved.L1, saved.L2, (), int(N), int(cut))
+        self.dtype = dtype or saved.dtype
+        self.sites = []
+        for l1 in range(saved.L1):
+            row = []
+            for l2 in range(saved.L2):
+                t = saved.site(l1, l2).to_tensor()
+                if t.dtype != self.dtype:
+                    t = t.to(self.dtype)
+                row.append(t)
+            self.sites.append(row)
+        self._projected = {}
+        return self
+
     def projected(self, l1, l2, occupation):
         """``_shrink_configuration`` (lattice.py:291-339): contract P0 with a
         one-hot shrinker; the remaining dimension-one leg is P_{l1}_{l2}_0.
