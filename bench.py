@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=256, help="configurations per lockstep pass of one lane")
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--lanes", type=int, default=4, help="concurrent lockstep lanes (host thread + HIP stream each)")
+    ap.add_argument("--lanes", type=int, default=8, help="concurrent lockstep lanes (host thread + HIP stream each)")
     ap.add_argument("--workload", default="C4", choices=["C2", "C3", "C4"])
     ap.add_argument("--cpu-configs", type=int, default=4, help="configs per host process for the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0)
@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="contraction GEMM precision of the timed engine")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile per size class")
+    ap.add_argument("--traffic-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (the reference kernel of known traffic that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--ess", default="", help="also measure ESS/sec of plain vs learned sampling on this workload")
     ap.add_argument("--ess-chains", type=int, default=256)
     ap.add_argument("--ess-diagonal", default="keep", choices=["keep", "hole"],
@@ -167,6 +170,7 @@ def sweep_timing(args, log, eng=None):
         eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.lanes)
     cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
+    eng.trim()
     smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
     smp.sweep(log=lambda m: log(f"sweep (warm-up): {m}"))
     log("sweep: warm-up sweep done")
@@ -435,6 +439,12 @@ def main():
     if args.direct:
         print(json.dumps({"direct_sampling": direct_timing(args, log)}), flush=True)
         return
+    if args.traffic_calibration:
+        src = torch.empty(2**27, dtype=torch.float64, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+        log("traffic calibration: 1 GiB device-to-device copy done")
     spec = getattr(peps, args.workload)
     storages = peps.random_storages(spec)
     lat = peps.Lattice(spec, storages)

@@ -144,7 +144,8 @@ int tns_plans_workspace(int n, tns_plan* const* plans, int dtype, int64_t* bytes
  * device data; name_ids: P0 T U D L R L1 L2 L3 R1 R2 P Q then the L1*L2
  * projected physical names P_{l1}_{l2}_0 row-major.  occupations: nconf x
  * L1 x L2 indices along the physical edge.  psi: host output (nconf).
- * The engine owns its device arenas until tns_engine_free. */
+ * The engine owns its device arenas until tns_engine_free (tns_engine_trim
+ * frees the working ones early). */
 typedef struct tns_engine tns_engine;
 int tns_engine_create(int kind, int L1, int L2, const tns_shape* const* sites, const void* const* site_data,
                       int dtype, const int32_t* name_ids, int n_names, int threads, tns_engine** out);
@@ -185,6 +186,12 @@ int tns_engine_set_profile(tns_engine* e, int on);
  * block structure), e.g. a gradient step (sampling_lattice/lattice.py:936-948):
  * the projected sites are rebuilt on the next evaluation */
 int tns_engine_refresh_sites(tns_engine* e);
+/* frees the engine's working device memory (boundary-MPS rows, temporaries,
+ * singular values; the projected sites stay): it grows again on the next
+ * evaluation.  No counterpart in the reference (TAT allocates per tensor);
+ * for callers that share the GPU's memory with other work between
+ * evaluations */
+int tns_engine_trim(tns_engine* e);
 int tns_engine_kernel_stats(const tns_engine* e, double* values, int n);
 int tns_engine_free(tns_engine* e);
 

@@ -121,3 +121,19 @@ def test_fp32_gemms_c3():
     d = np.abs(np.log(np.abs(got)) - np.log(np.abs(ref)))
     print("C3 fp32 max |dlogpsi|", d.max(), "median", np.median(d))
     assert d.max() < BOUND8["fp32"], d
+
+
+def test_trim_keeps_psi():
+    """Freeing the working arenas between evaluations changes nothing: the
+    next evaluation regrows them and gives bit-identical psi (two lanes)."""
+    from tnsp_amd import peps as ppeps
+    from tnsp_amd.engine import AmplitudeEngine
+
+    spec, storages, configs, _ = c2_case()
+    eng = AmplitudeEngine(ppeps.Lattice(spec, storages), chunk=8, lanes=2)
+    first = eng.evaluate(configs)
+    eng.trim()
+    again = eng.evaluate(configs)
+    eng.trim()
+    eng.trim()
+    assert np.array_equal(first, again) and np.array_equal(eng.evaluate(configs[:3]), first[:3])

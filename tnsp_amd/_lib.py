@@ -76,6 +76,7 @@ _SIGNATURES = {
     "tns_engine_reset_stats": [c_vp],
     "tns_engine_set_profile": [c_vp, ctypes.c_int],
     "tns_engine_refresh_sites": [c_vp],
+    "tns_engine_trim": [c_vp],
     "tns_engine_kernel_stats": [c_vp, c_dblp, ctypes.c_int],
     "tns_engine_free": [c_vp],
     "tns_conv3x3_forward": [c_vp, c_vp, c_vp, c_vp] + [ctypes.c_int] * 6 + [c_vp],

@@ -93,7 +93,9 @@ void Pool::parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& 
 // ------------------------------------------------------------------ arena
 void DevArena::ensure(size_t keep, hipStream_t st) {
     if (used <= cap) return;
-    size_t ncap = std::max(used + (used >> 2), cap * 2);
+    // 1.25x what is used, at least 1.5x the old capacity: the arenas reach
+    // their working size in a few steps without doubling past it
+    size_t ncap = std::max(used + (used >> 2), cap + (cap >> 1));
     ncap = (ncap + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
     char* nb = nullptr;
     check_hip(hipMalloc(reinterpret_cast<void**>(&nb), ncap), "engine arena alloc");
@@ -128,6 +130,12 @@ Engine::Engine(int kind, int L1, int L2, const std::vector<Shape>& sites, const 
 
 Engine::~Engine() {
     for (auto& a : arenas_) a.release();
+}
+
+void Engine::trim() {
+    // the working arenas regrow on the next evaluation; the projected sites stay
+    for (int i = 0; i < NARENA; i++)
+        if (i != A_PERSIST) arenas_[i].release();
 }
 
 void Engine::grow_all(hipStream_t st) {
@@ -863,6 +871,12 @@ int tns_engine_stats(const tns_engine* e, double* values, int n) {
 int tns_engine_reset_stats(tns_engine* e) {
     TNS_TRY({
         for (auto& l : e->lanes) l->reset_stats();
+    })
+}
+
+int tns_engine_trim(tns_engine* e) {
+    TNS_TRY({
+        for (auto& l : e->lanes) l->trim();
     })
 }
 

@@ -115,6 +115,8 @@ class Engine {
     void set_profile(bool on) { profile_ = on; }
     // the raw site buffers changed in place: project again on the next call
     void refresh_sites() { proj_.clear(); }
+    // frees the working arenas (every one but the projected sites)
+    void trim();
     const KernelTimer& timer() const { return timer_; }
 
    private:

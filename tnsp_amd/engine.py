@@ -87,6 +87,11 @@ class AmplitudeEngine:
         project them again on the next evaluation."""
         _lib.check(_lib.lib().tns_engine_refresh_sites(self.handle))
 
+    def trim(self):
+        """Free the working device memory (it grows again on the next
+        evaluation; the projected sites stay)."""
+        _lib.check(_lib.lib().tns_engine_trim(self.handle))
+
     def stats(self):
         vals = (ctypes.c_double * len(STAT_NAMES))()
         _lib.check(_lib.lib().tns_engine_stats(self.handle, vals, len(STAT_NAMES)))

@@ -176,6 +176,9 @@ class EnvironmentSampler:
         self._shr = [[(self.lattice.shrinker(l1, l2, 0), self.lattice.shrinker(l1, l2, 1))
                       for l2 in range(self.L2)] for l1 in range(self.L1)]
         self.psi = np.abs(engine.evaluate(self.configs, cut=self.cut))
+        # the refreshes evaluate a few configurations at a time: give the
+        # memory of this full batch back to the sampler's environments
+        engine.trim()
         if np.any(self.psi == 0):
             raise ValueError("initial configurations with psi = 0")
         self.proposals = 0
