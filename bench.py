@@ -490,6 +490,9 @@ def main():
     eng.set_profile(False)
     ks = eng.kernel_stats()
     eng.set_lanes(args.lanes)
+    # the legs below run their own samplers and engines: give this engine's
+    # working memory back first
+    eng.trim()
     print(f"[bench] kernel timing pass: {min(B, args.chunk)} configurations on one lane", file=sys.stderr,
           flush=True)
     dom = dominant_kernel(ks)

@@ -32,8 +32,12 @@ amplitude computed afresh, and the draw compares it with the carried psi.
 boundary-MPS truncation, so the step accepts with probability one up to
 that difference).
 
-Step by step, each lane refreshes hole(()) with an amplitude engine of its
-own, so concurrent lanes do not queue on one engine handle.
+Step by step, a refresh in the horizontal pass at row l1 restarts from the
+boundary MPS of the rows below l1 built at the sweep start (they have not
+moved since), as the lazy graph recomputes only the invalidated nodes; in
+the vertical pass every row may have moved, and each lane evaluates the
+amplitude with an engine of its own, so concurrent lanes do not queue on
+one engine handle.
 
 A hole(()) changes only the carried psi, never the configuration, so no
 environment depends on it: only later acceptance tests read it.  With
@@ -196,6 +200,7 @@ class EnvironmentSampler:
         # step-by-step refreshes of concurrent lanes do not queue on one handle
         self._lane = 0
         self._lane_engines = {}
+        self._hole_rows = None
         from .sampler import _fermion_order_index, hole_name_order
 
         self._hole_pos = {n: i for i, n in enumerate(hole_name_order(self.L1, self.L2))}
@@ -289,12 +294,29 @@ class EnvironmentSampler:
             # last known, as the lazy graph's cache does
             need = dbl[np.isnan(self._hole_cache[dbl])]
             if len(need):
-                self._hole_cache[need] = np.abs(self._engine_of_lane().evaluate(self.configs[need], cut=self.cut))
+                if self._hole_rows is not None:
+                    self._hole_cache[need] = self._hole_from_rows(need, *self._hole_rows)
+                else:
+                    self._hole_cache[need] = np.abs(self._engine_of_lane().evaluate(self.configs[need], cut=self.cut))
                 self.refreshed += len(need)
             wss = self._hole_cache[dbl]
             with np.errstate(divide="ignore", invalid="ignore"):
                 acc = u[dbl] < (wss / self.psi[dbl]) ** (2 * self.alpha) * n[dbl] / ns[dbl]
             self.psi[dbl[acc]] = wss[acc]
+
+    def _hole_from_rows(self, idx, l1, down):
+        """|hole(())| of chains ``idx`` during the horizontal pass at row l1:
+        _inline_right_to_left[0, 0] over _down_to_up[1], whose rows below l1
+        are unchanged since ``down`` was built at the sweep start, so the
+        chain restarts from down[l1 + 1] (the lazy graph recomputes only the
+        nodes the moved sites invalidated: rows l1 .. 1 and row 0)."""
+        L2 = self.L2
+        one = self._ones(len(idx))
+        d = [t.take(idx) for t in down[l1 + 1]]
+        for k in range(l1, 0, -1):
+            d = self._two_line("DULR", d, [self._site(k, c, self.configs[idx, k, c]) for c in range(L2)])
+        row0 = [self._site(0, c, self.configs[idx, 0, c]) for c in range(L2)]
+        return np.abs(self._right_tailed(row0, [one] * L2, d, one)[0].values())
 
     def _engine_of_lane(self):
         """The amplitude engine this lane evaluates hole(()) with: the
@@ -358,6 +380,8 @@ class EnvironmentSampler:
         down = self._down_mps(one)
         up = [one] * L2
         for l1 in range(L1):
+            # rows below l1 are as they were when down was built
+            self._hole_rows = (l1, down)
             dn = down[l1 + 1]
             row = self._row(l1)
             rt = self._right_tailed(row, up, dn, one)
@@ -388,6 +412,7 @@ class EnvironmentSampler:
                 up = self._two_line("UDLR", up, self._row(l1))
             if log:
                 log(f"row {l1} done")
+        self._hole_rows = None
 
     def _vertical(self, log=None, observe=False):
         L1, L2 = self.L1, self.L2
