@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--chunk", type=int, default=256, help="configurations per lockstep pass of one lane")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=8, help="concurrent lockstep lanes (host thread + HIP stream each)")
+    ap.add_argument("--sampler-lanes", type=int, default=4,
+                    help="lanes of the sampling legs (sweep, ESS, direct, gradient descent): their batched ops "
+                         "split the chains, so fewer, larger lanes do better there")
     ap.add_argument("--workload", default="C4", choices=["C2", "C3", "C4"])
     ap.add_argument("--cpu-configs", type=int, default=4, help="configs per host process for the CPU baseline")
     ap.add_argument("--cpu-procs", type=int, default=0)
@@ -101,10 +104,10 @@ def vmc_run(args, log):
 
     spec = getattr(peps, args.vmc)
     lat = peps.Lattice(spec)
-    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.sampler_lanes)
     cand = peps.sample_configurations(spec, 4 * args.vmc_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.vmc_chains]
-    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
+    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.sampler_lanes)
     direct = None
     if args.vmc_direct:
         from tnsp_amd.direct import DirectSampler
@@ -112,7 +115,7 @@ def vmc_run(args, log):
         direct = DirectSampler(lat, cut2=args.direct_cut2, seed=2)
     recs = gradient_descent(smp, args.vmc_steps, args.vmc_step_size, sweeps=args.vmc_sweeps,
                             natural=args.vmc_natural, log=log, direct=direct, direct_samples=args.vmc_direct)
-    return {"workload": args.vmc, "chains": len(start), "lanes": args.lanes, "natural": args.vmc_natural,
+    return {"workload": args.vmc, "chains": len(start), "lanes": args.sampler_lanes, "natural": args.vmc_natural,
             "step_size": args.vmc_step_size, "sweeps_per_step": args.vmc_sweeps,
             "direct_samples_per_step": args.vmc_direct, "direct_cut2": args.direct_cut2 if direct else None,
             "instability": [r.get("instability") for r in recs],
@@ -146,7 +149,7 @@ def direct_timing(args, log):
     poss, cfg = dsm.sample(args.direct_samples, log=lambda m: log(f"direct: {m}"))
     torch.cuda.synchronize()
     dt = time.time() - t0
-    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.sampler_lanes)
     w = reweight(eng.evaluate(cfg), poss)
     return {"workload": args.direct, "samples": args.direct_samples, "cut": spec.cut, "cut2": args.direct_cut2,
             "double_layer_s": t_double, "draw_s": dt, "samples_per_s": args.direct_samples / dt,
@@ -167,17 +170,18 @@ def sweep_timing(args, log, eng=None):
 
     spec = getattr(peps, args.sweep)
     if eng is None:
-        eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.lanes)
+        eng = AmplitudeEngine(peps.Lattice(spec), chunk=256, lanes=args.sampler_lanes)
     cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
     eng.trim()
-    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
-    smp.sweep(log=lambda m: log(f"sweep (warm-up): {m}"))
+    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.sampler_lanes)
+    t0 = time.time()
+    smp.sweep(log=lambda m: log(f"sweep (warm-up): {m} ({time.time() - t0:.1f} s)"))
     log("sweep: warm-up sweep done")
     torch.cuda.synchronize()
     t0 = time.time()
     for k in range(args.sweep_count):
-        smp.sweep(log=lambda m: log(f"sweep {k + 1}: {m}"))
+        smp.sweep(log=lambda m: log(f"sweep {k + 1}: {m} ({time.time() - t0:.1f} s)"))
         log(f"sweep: {k + 1}/{args.sweep_count} timed sweeps")
     torch.cuda.synchronize()
     dt = (time.time() - t0) / args.sweep_count
@@ -185,7 +189,7 @@ def sweep_timing(args, log, eng=None):
     e = smp.local_energy()
     torch.cuda.synchronize()
     de = time.time() - t0
-    return {"workload": args.sweep, "chains": len(start), "lanes": args.lanes, "diagonal": args.ess_diagonal,
+    return {"workload": args.sweep, "chains": len(start), "lanes": args.sampler_lanes, "diagonal": args.ess_diagonal,
             # "keep" skips tetragono's hole(()) refresh of doubly occupied
             # bonds: a cheaper chain than the reference's at a truncating cut
             "reference_chain": args.ess_diagonal == "hole",
@@ -345,12 +349,12 @@ def ess_comparison(args, log):
     lat = peps.Lattice(spec)
     # 256 configurations per lane chunk: at C4 the arenas of one chunk hold
     # about 30 GB, so four lanes of 1024 would not fit in HBM
-    eng = AmplitudeEngine(lat, chunk=256, lanes=args.lanes)
+    eng = AmplitudeEngine(lat, chunk=256, lanes=args.sampler_lanes)
     L = spec.L1 * spec.L2
     cand = peps.sample_configurations(spec, 4 * args.ess_chains, seed=77)
     psi = eng.evaluate(cand)
     start = cand[psi != 0][:args.ess_chains]
-    sweep = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.lanes)
+    sweep = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.sampler_lanes)
 
     def energy(configs):
         # tetragono's observer: psi(s') by replace over cached environments
@@ -596,8 +600,11 @@ def main():
             "gain": (ess or ess_leg)["gain"], "gain_second_observable": (ess or ess_leg)["gain_density"],
             "reference_chain": (ess or ess_leg)["sweep"]["reference_chain"],
             "note": "ESS of the tV energy estimator per second of sampling (t = V = 1); the sweep is the "
-                    "cached-environment replace sweep; reference_chain false: doubly occupied bonds keep "
-                    "psi instead of tetragono's hole(()) refresh"},
+                    "cached-environment replace sweep" + (
+                        ", tetragono's chain (hole(()) refresh of doubly occupied bonds)"
+                        if (ess or ess_leg)["sweep"]["reference_chain"] else
+                        "; reference_chain false: doubly occupied bonds keep psi instead of tetragono's "
+                        "hole(()) refresh (a cheaper, different chain)")},
         "extra": {"kernel_timing": f"one lane, {min(B, args.chunk)} configurations, HIP events on the launch stream",
                   "kernel_ms": {k: v["ms"] for k, v in ks.items()},
                   "kernel_rates": kernel_rates(ks),
