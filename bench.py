@@ -74,6 +74,9 @@ def parse():
                                                 "its observer energy on this workload")
     ap.add_argument("--sweep-only", action="store_true", help="time only the sweep and observer (one JSON line)")
     ap.add_argument("--sweep-chains", type=int, default=256)
+    ap.add_argument("--sweep-diagonal", default="hole", choices=["keep", "hole"],
+                    help="doubly occupied bonds in the sweep timing: tetragono's hole(()) refresh (the reference "
+                         "chain) or keep psi")
     ap.add_argument("--sweep-count", type=int, default=1, help="timed sweeps after one untimed sweep")
     ap.add_argument("--no-sweep", action="store_true", help="skip the sweep timing of the default run")
     ap.add_argument("--vmc", default="", help="run tetragono-style gradient descent on this workload (one JSON line)")
@@ -174,7 +177,7 @@ def sweep_timing(args, log, eng=None):
     cand = peps.sample_configurations(spec, 4 * args.sweep_chains, seed=77)
     start = cand[eng.evaluate(cand) != 0][:args.sweep_chains]
     eng.trim()
-    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.ess_diagonal, lanes=args.sampler_lanes)
+    smp = EnvironmentSampler(eng, start, seed=1, diagonal=args.sweep_diagonal, lanes=args.sampler_lanes)
     t0 = time.time()
     smp.sweep(log=lambda m: log(f"sweep (warm-up): {m} ({time.time() - t0:.1f} s)"))
     log("sweep: warm-up sweep done")
@@ -189,10 +192,10 @@ def sweep_timing(args, log, eng=None):
     e = smp.local_energy()
     torch.cuda.synchronize()
     de = time.time() - t0
-    return {"workload": args.sweep, "chains": len(start), "lanes": args.sampler_lanes, "diagonal": args.ess_diagonal,
+    return {"workload": args.sweep, "chains": len(start), "lanes": args.sampler_lanes, "diagonal": args.sweep_diagonal,
             # "keep" skips tetragono's hole(()) refresh of doubly occupied
             # bonds: a cheaper chain than the reference's at a truncating cut
-            "reference_chain": args.ess_diagonal == "hole",
+            "reference_chain": args.sweep_diagonal == "hole",
             "sweep_s": dt, "chain_sweeps_per_s": len(start) / dt, "bonds": len(smp.bonds),
             "proposals_per_s": smp.proposals / (args.sweep_count + 1) / dt,
             "acceptance": smp.accepted / max(1, smp.proposals), "hole_refreshes": smp.refreshed,
