@@ -21,6 +21,19 @@ namespace {
 constexpr int WG = 256;
 constexpr int WAVE = 64;
 
+// Device pointers read from task records are generic to the compiler, so
+// accesses through them compile to flat instructions, and a flat load is
+// counted by the LDS counter as well as the vector-memory one: a wait for an
+// LDS read then also waits for every global load in flight (the prefetch of
+// a pipelined loop included).  Every task buffer is global memory; accesses
+// through these pointers compile to global_load / global_store.
+template<typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template<typename T>
+__device__ inline gptr<T> to_global(T* p) {
+    return (gptr<T>)(p);
+}
+
 template<typename T>
 __device__ inline T neg_if(T v, int neg) {
     return neg ? -v : v;
@@ -78,8 +91,8 @@ __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__
             mul[r] = tk.mul[r];
             sh[r] = tk.sh[r];
         }
-        const TS* src = static_cast<const TS*>(tk.src);
-        T* dst = static_cast<T*>(tk.dst);
+        const gptr<const TS> src = to_global(static_cast<const TS*>(tk.src));
+        const gptr<T> dst = to_global(static_cast<T*>(tk.dst));
         const bool neg = tk.neg != 0;
         const int64_t left = p1 - base;  // elements of task t0 from base on
         const uint32_t off = uint32_t(base - p0);
@@ -99,8 +112,8 @@ __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__
         return;
     }
     int t = t0;
-    const TS* src[PER];
-    T* dst[PER];
+    gptr<const TS> src[PER];
+    gptr<T> dst[PER];
     int64_t so[PER], dof[PER];
     bool neg[PER], ok[PER];
     int64_t end = p1;
@@ -120,8 +133,8 @@ __global__ __launch_bounds__(WG) void permute_kernel(const PermDev* __restrict__
         if (ok[j]) {
             const PermDev& tk = tasks[t];
             perm_decode(uint32_t(e - prefix[t]), tk.rank, tk.dims, tk.ss, tk.ds, tk.mul, tk.sh, so[j], dof[j]);
-            src[j] = static_cast<const TS*>(tk.src);
-            dst[j] = static_cast<T*>(tk.dst);
+            src[j] = to_global(static_cast<const TS*>(tk.src));
+            dst[j] = to_global(static_cast<T*>(tk.dst));
             neg[j] = tk.neg != 0;
         }
     }
@@ -146,7 +159,7 @@ __global__ __launch_bounds__(WG) void fill_kernel(const FillDev* __restrict__ ta
             if (t >= wg_first[gridDim.x]) return;
             end = prefix[t + 1];
         }
-        static_cast<T*>(tasks[t].p)[e - prefix[t]] = T(0);
+        to_global(static_cast<T*>(tasks[t].p))[e - prefix[t]] = T(0);
     }
 }
 
@@ -218,8 +231,8 @@ __global__ __launch_bounds__(64 * WR * WC) void gemm_kernel(const GemmDev* __res
     __shared__ T Bs[2][KB][TN + 1];
     const TileDev tile = tiles[blockIdx.x];
     const GemmDev g = tasks[tile.task];
-    const T* A = static_cast<const T*>(g.a);
-    const T* B = static_cast<const T*>(g.b);
+    const gptr<const T> A = to_global(static_cast<const T*>(g.a));
+    const gptr<const T> B = to_global(static_cast<const T*>(g.b));
     const int row0 = tile.tm * TM, col0 = tile.tn * TN;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const int wr = (wave / WC) * 16 * FM, wc = (wave % WC) * 16 * FN;
@@ -312,7 +325,7 @@ __global__ __launch_bounds__(64 * WR * WC) void gemm_kernel(const GemmDev* __res
         if (kb + 1 < nk) store(cur ^ 1);
         __syncthreads();
     }
-    T* C = static_cast<T*>(g.c);
+    const gptr<T> C = to_global(static_cast<T*>(g.c));
     const T alpha = T(g.alpha);
 #pragma unroll
     for (int fi = 0; fi < FM; fi++)
@@ -337,8 +350,8 @@ __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict
     __shared__ float Bs[KB][BT + 1];
     const TileDev tile = tiles[blockIdx.x];
     const GemmDev g = tasks[tile.task];
-    const float* A = static_cast<const float*>(g.a);
-    const float* B = static_cast<const float*>(g.b);
+    const gptr<const float> A = to_global(static_cast<const float*>(g.a));
+    const gptr<const float> B = to_global(static_cast<const float*>(g.b));
     const int row0 = tile.tm * BT, col0 = tile.tn * BT;
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const int wr = (wave / 2) * 16, wc = (wave % 2) * 16;
@@ -387,7 +400,7 @@ __global__ __launch_bounds__(WG) void gemm_kernel_bf16(const GemmDev* __restrict
         }
         __syncthreads();
     }
-    float* C = static_cast<float*>(g.c);
+    const gptr<float> C = to_global(static_cast<float*>(g.c));
     const float alpha = float(g.alpha);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -556,13 +569,27 @@ __device__ inline double refined_rsqrt(double y) {
     return r * fma(-0.5 * y * r, r, 1.5);
 }
 
-__device__ inline void jacobi_rotation(double saa, double sbb, double sab, double& cs, double& sn) {
+// t = tan of the angle: after a' = cs a - sn b, b' = sn a + cs b the squared
+// norms are exactly saa - t sab and sbb + t sab (t solves t^2 + 2 zeta t = 1)
+__device__ inline void jacobi_rotation(double saa, double sbb, double sab, double& cs, double& sn, double& t) {
     const double zeta = (sbb - saa) / (2.0 * sab);
     // for |zeta| beyond 1e150, zeta^2 would overflow: t = 1 / (2 zeta) there
-    const double t = fabs(zeta) > 1e150 ? 0.5 / zeta
-                                        : copysign(refined_rcp(fabs(zeta) + sqrt(fma(zeta, zeta, 1.0))), zeta);
+    t = fabs(zeta) > 1e150 ? 0.5 / zeta : copysign(refined_rcp(fabs(zeta) + sqrt(fma(zeta, zeta, 1.0))), zeta);
     cs = refined_rsqrt(fma(t, t, 1.0));
     sn = cs * t;
+}
+
+__device__ inline void jacobi_rotation(double saa, double sbb, double sab, double& cs, double& sn) {
+    double t;
+    jacobi_rotation(saa, sbb, sab, cs, sn, t);
+}
+
+// Squared column norm after a rotation, tracked instead of recomputed: the
+// update keeps its rounding relative to the old norm, so a column that lost
+// all but 1e-3 of it is marked unknown (-1: no rotation until the next
+// exact refresh) rather than carrying a large relative error.
+__device__ inline double jacobi_tracked(double old_n, double new_n) {
+    return new_n < 1e-3 * old_n ? -1.0 : new_n;
 }
 
 // --------------------------------------------------------------------- qr
@@ -761,34 +788,51 @@ __device__ inline void qb_build_t(const double* V, int rows, int nb, const doubl
 // chunks of QB_WC columns on the fp64 MFMA (16x16x4): W = V^T A with rows as
 // the reduction dimension (one wave per 16-column tile), op(T) W per column,
 // then A -= V W computed as (W^T V^T) tiles of 16 columns x 16 rows so that
-// the stores run down the columns.
-__device__ inline void qb_apply(const double* V, int rows, int nb, const double* Tm, double* W, double* X, int mx,
-                                int p0, int c_begin, int c_end, bool trans) {
+// the stores run down the columns.  A lives in global memory and each wave
+// has few companions on its SIMD, so both passes keep many independent
+// loads in flight: 64 rows of A per step in the first pass, the A tiles of
+// four row tiles loaded before their MFMAs in the second.
+__device__ inline void qb_apply(const double* V, int rows, int nb, const double* Tm, double* W, gptr<double> X,
+                                int mx, int p0, int c_begin, int c_end, bool trans) {
     const int tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     constexpr int NW = QR_WG / WAVE;
+    constexpr int RS = 16;  // 4-row MFMA steps per iteration of the first pass
+    constexpr int RG = 4;   // 16-row tiles per task of the second pass
+    const int rows_full = rows - rows % (4 * RS);
     for (int cb = c_begin; cb < c_end; cb += QB_WC) {
         const int ncol = min(QB_WC, c_end - cb), ntile = (ncol + 15) / 16;
         for (int t = wave; t < ntile; t += NW) {
+            // lanes of columns beyond ncol (or reflectors beyond nb) read
+            // column 0 (reflector 0) instead: their results land in entries
+            // of W that are never stored
             const int c = t * 16 + lane % 16, j = lane % 16;
-            const double* xc = X + int64_t(cb + (c < ncol ? c : 0)) * mx + p0;
+            const gptr<const double> xc = X + int64_t(cb + (c < ncol ? c : 0)) * mx + p0;
             const double* vj = V + int64_t(j < nb ? j : 0) * rows;
+            const int rl = lane / 16;
             d4 acc = {0.0, 0.0, 0.0, 0.0};
-            // 16 rows per iteration: the four global loads are issued
-            // before the four MFMAs that consume them
-            for (int r0 = 0; r0 < rows; r0 += 16) {
+            for (int r0 = 0; r0 < rows_full; r0 += 4 * RS) {
+                double av[RS], bv[RS];
+#pragma unroll
+                for (int u = 0; u < RS; u++) bv[u] = xc[r0 + 4 * u + rl];
+#pragma unroll
+                for (int u = 0; u < RS; u++) av[u] = vj[r0 + 4 * u + rl];
+#pragma unroll
+                for (int u = 0; u < RS; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+            }
+            for (int r0 = rows_full; r0 < rows; r0 += 16) {
                 double av[4], bv[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int r = r0 + 4 * u + lane / 16;
-                    av[u] = (j < nb && r < rows) ? vj[r] : 0.0;
-                    bv[u] = (c < ncol && r < rows) ? xc[r] : 0.0;
+                    const int r = r0 + 4 * u + rl;
+                    av[u] = r < rows ? vj[r] : 0.0;
+                    bv[u] = r < rows ? xc[r] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
             }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int jj = lane / 16 + 4 * i;
+                const int jj = rl + 4 * i;
                 if (jj < nb && c < ncol) W[jj * QB_WC + c] = acc[i];
             }
         }
@@ -808,21 +852,42 @@ __device__ inline void qb_apply(const double* V, int rows, int nb, const double*
             }
         }
         __syncthreads();
-        const int nrt = (rows + 15) / 16;
-        for (int q = wave; q < nrt * ntile; q += NW) {
-            const int rt = q / ntile, ct = q % ntile;
-            const int ca = ct * 16 + lane % 16, rb = rt * 16 + lane % 16;
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < nb; k0 += 4) {
-                const int jk = k0 + lane / 16;
-                const double a = (jk < nb && ca < ncol) ? W[jk * QB_WC + ca] : 0.0;
-                const double b = (jk < nb && rb < rows) ? V[int64_t(jk) * rows + rb] : 0.0;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        const int nrt = (rows + 15) / 16, nrg = (nrt + RG - 1) / RG;
+        for (int q = wave; q < nrg * ntile; q += NW) {
+            const int rg = q / ntile, ct = q % ntile;
+            const int ca = ct * 16 + lane % 16;
+            double xv[RG][4];
+#pragma unroll
+            for (int g = 0; g < RG; g++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = ct * 16 + lane / 16 + 4 * i, r = (rg * RG + g) * 16 + lane % 16;
+                    xv[g][i] = (c < ncol && r < rows) ? X[int64_t(cb + c) * mx + p0 + r] : 0.0;
+                }
+            }
+            double wa[QB_NB / 4];
+#pragma unroll
+            for (int s4 = 0; s4 < QB_NB / 4; s4++) {
+                const int jk = 4 * s4 + lane / 16;
+                wa[s4] = (jk < nb && ca < ncol) ? W[jk * QB_WC + ca] : 0.0;
             }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int c = ct * 16 + lane / 16 + 4 * i, r = rt * 16 + lane % 16;
-                if (c < ncol && r < rows) X[int64_t(cb + c) * mx + p0 + r] -= acc[i];
+            for (int g = 0; g < RG; g++) {
+                const int rb = (rg * RG + g) * 16 + lane % 16;
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s4 = 0; s4 < QB_NB / 4; s4++) {
+                    if (4 * s4 < nb) {
+                        const int jk = 4 * s4 + lane / 16;
+                        const double b = (jk < nb && rb < rows) ? V[int64_t(jk) * rows + rb] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(wa[s4], b, acc, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int c = ct * 16 + lane / 16 + 4 * i, r = rb;
+                    if (c < ncol && r < rows) X[int64_t(cb + c) * mx + p0 + r] = xv[g][i] - acc[i];
+                }
             }
         }
         __syncthreads();
@@ -844,9 +909,9 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
     double* W = Gm + QB_NB * QB_NB;
     double* taus = W + QB_NB * QB_WC;
     double* V = taus + QB_NB;
-    double* X = tk.work;
-    double* tau = X + int64_t(mx) * nx;
-    const T* in = static_cast<const T*>(tk.in);
+    const gptr<double> X = to_global(tk.work);
+    const gptr<double> tau = X + int64_t(mx) * nx;
+    const gptr<const T> in = to_global(static_cast<const T*>(tk.in));
     const int tid = threadIdx.x;
     const int G = 32, NG = QR_WG / G, g = tid / G, gl = tid % G;
     for (int64_t e = tid; e < int64_t(mx) * nx; e += QR_WG) {
@@ -917,8 +982,8 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
         }
         __syncthreads();
     }
-    T* f1 = static_cast<T*>(tk.f1);
-    T* f2 = static_cast<T*>(tk.f2);
+    const gptr<T> f1 = to_global(static_cast<T*>(tk.f1));
+    const gptr<T> f2 = to_global(static_cast<T*>(tk.f2));
     if (!lq) {
         for (int64_t e = tid; e < int64_t(k) * n; e += QR_WG) {
             const int r = int(e / n), c = int(e % n);
@@ -1339,76 +1404,92 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
     // rotation of the pair (x of column cx, y of column cy) as seen from the
     // lane holding x: both lanes of a duplicated pair order it by column
     // index, so they compute the same rotation and each keeps its own column
-    auto cross_round = [&]() {
-        double saa = 0, sbb = 0, sab = 0, saa1 = 0, sbb1 = 0, sab1 = 0;
+    // squared norms of the lane's two columns: exact after refresh(), then
+    // carried through the rotations (and the bottoms' moves) of one block
+    // round, so that a pair costs one dot product instead of three
+    double na = 0.0, nb = 0.0;
+    auto refresh = [&]() __attribute__((always_inline)) {
+        double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
 #pragma unroll
         for (int i = 0; i < RPT; i += 2) {
-            saa += xa[i] * xa[i];
-            sbb += xb[i] * xb[i];
+            a0 += xa[i] * xa[i];
+            b0 += xb[i] * xb[i];
+            a1 += xa[i + 1] * xa[i + 1];
+            b1 += xb[i + 1] * xb[i + 1];
+        }
+        na = sum8(a0 + a1);
+        nb = sum8(b0 + b1);
+    };
+    auto cross_round = [&]() __attribute__((always_inline)) {
+        double sab = 0, sab1 = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i += 2) {
             sab += xa[i] * xb[i];
-            saa1 += xa[i + 1] * xa[i + 1];
-            sbb1 += xb[i + 1] * xb[i + 1];
             sab1 += xa[i + 1] * xb[i + 1];
         }
-        saa = sum8(saa + saa1);
-        sbb = sum8(sbb + sbb1);
         sab = sum8(sab + sab1);
+        const double saa = na, sbb = nb;
         if (ca < kx && cb < kx && saa > live && sbb > live && sab != 0.0 && fabs(sab) > tol * sqrt(saa) * sqrt(sbb)) {
-            double cs, sn;
-            jacobi_rotation(saa, sbb, sab, cs, sn);
+            double cs, sn, t;
+            jacobi_rotation(saa, sbb, sab, cs, sn, t);
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
                 const double u = xa[i], v = xb[i];
                 xa[i] = cs * u - sn * v;
                 xb[i] = sn * u + cs * v;
             }
+            na = jacobi_tracked(saa, saa - t * sab);
+            nb = jacobi_tracked(sbb, sbb + t * sab);
             rot = 1;
         }
+    };
+    auto move_b = [&](auto dtag) __attribute__((always_inline)) {
+        constexpr int D = decltype(dtag)::value;
+        move_xor<D>(xb, cb);
+        nb = xor_slot<D>(nb);
     };
     // 8 cross rounds: top slot p meets bottom slot p ^ r in round r (the
     // bottoms move by a DPP lane permutation within each 8-lane group between
     // rounds: xor 1 and 3 by quad permutes, xor 7 by the half-row mirror), so
     // each top meets each bottom of the wave once
-    auto cross8 = [&]() {
+    auto cross8 = [&]() __attribute__((always_inline)) {
+        using X1 = std::integral_constant<int, 1>;
+        using X3 = std::integral_constant<int, 3>;
+        using X7 = std::integral_constant<int, 7>;
         cross_round();
-        move_xor<1>(xb, cb);
+        move_b(X1());
         cross_round();
-        move_xor<3>(xb, cb);
+        move_b(X3());
         cross_round();
-        move_xor<1>(xb, cb);
+        move_b(X1());
         cross_round();
-        move_xor<7>(xb, cb);
+        move_b(X7());
         cross_round();
-        move_xor<1>(xb, cb);
+        move_b(X1());
         cross_round();
-        move_xor<3>(xb, cb);
+        move_b(X3());
         cross_round();
-        move_xor<1>(xb, cb);
+        move_b(X1());
         cross_round();
     };
     // intra round: slot p meets slot p ^ D among the tops and among the
     // bottoms; both slots of a pair hold it (each updates its own column)
-    auto intra = [&](auto dtag) {
+    auto intra = [&](auto dtag) __attribute__((always_inline)) {
         constexpr int D = decltype(dtag)::value;
         // the partner's rows are fetched by DPP twice (for the sums, then for
         // the update) instead of being held: no extra registers
         const int pa = xor_slot_int<D>(ca), pb = xor_slot_int<D>(cb);
-        double s0 = 0, s1 = 0, s2 = 0, t0 = 0, t1 = 0, t2 = 0;
+        // own and partner squared norms (the partner's tracked value moves
+        // by the same permutation as its rows, so both lanes see one pair)
+        const double s0 = na, s1 = xor_slot<D>(na), t0 = nb, t1 = xor_slot<D>(nb);
+        double s2 = 0, t2 = 0;
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             const double ya = xor_slot<D>(xa[i]), yb = xor_slot<D>(xb[i]);
-            s0 += xa[i] * xa[i];
-            s1 += ya * ya;
             s2 += xa[i] * ya;
-            t0 += xb[i] * xb[i];
-            t1 += yb * yb;
             t2 += xb[i] * yb;
         }
-        s0 = sum8(s0);
-        s1 = sum8(s1);
         s2 = sum8(s2);
-        t0 = sum8(t0);
-        t1 = sum8(t1);
         t2 = sum8(t2);
         // tops: (lower index, higher index) order fixes the rotation, the
         // lane keeps its own new column
@@ -1418,11 +1499,12 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
             const bool low = ca < pa;
             const double saa = low ? s0 : s1, sbb = low ? s1 : s0;
             if (ca < kx && pa < kx && saa > live && sbb > live && s2 != 0.0 && fabs(s2) > tol * sqrt(saa) * sqrt(sbb)) {
-                double cs, sn;
-                jacobi_rotation(saa, sbb, s2, cs, sn);
+                double cs, sn, t;
+                jacobi_rotation(saa, sbb, s2, cs, sn, t);
                 // own' = cs own - sn partner (low) or sn partner + cs own
                 ca_c = cs;
                 ca_s = low ? -sn : sn;
+                na = low ? jacobi_tracked(saa, saa - t * s2) : jacobi_tracked(sbb, sbb + t * s2);
                 ra = true;
             }
         }
@@ -1430,10 +1512,11 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
             const bool low = cb < pb;
             const double saa = low ? t0 : t1, sbb = low ? t1 : t0;
             if (cb < kx && pb < kx && saa > live && sbb > live && t2 != 0.0 && fabs(t2) > tol * sqrt(saa) * sqrt(sbb)) {
-                double cs, sn;
-                jacobi_rotation(saa, sbb, t2, cs, sn);
+                double cs, sn, t;
+                jacobi_rotation(saa, sbb, t2, cs, sn, t);
                 cb_c = cs;
                 cb_s = low ? -sn : sn;
+                nb = low ? jacobi_tracked(saa, saa - t * t2) : jacobi_tracked(sbb, sbb + t * t2);
                 rb = true;
             }
         }
@@ -1449,6 +1532,7 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
         rot = 0;
         // block round 0: the pairs inside each half-block, then tops with
         // bottoms
+        refresh();
         intra(std::integral_constant<int, 1>());
         intra(std::integral_constant<int, 2>());
         intra(std::integral_constant<int, 3>());
@@ -1497,6 +1581,7 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
                 ca = ibuf[NWV][pl];
             }
             __syncthreads();
+            refresh();
             cross8();
         }
         if (!__syncthreads_or(rot)) {
@@ -1682,13 +1767,20 @@ void launch_gemm(const GemmDev* tasks, const TileDev* tiles, int64_t num_tiles, 
 }
 
 namespace {
-template<typename K>
-void allow_lds(K kernel, int64_t bytes) {
-    static int64_t granted[4] = {0, 0, 0, 0};
-    (void)granted;
-    if (bytes > 64 * 1024)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  int(bytes));
+// Raise a kernel's dynamic-LDS limit to the most any of its launches uses,
+// once per kernel and process: the static's initialisation is thread-safe,
+// so lanes launching from several host threads never call into the runtime
+// for it concurrently or per launch.
+template<auto Kernel>
+void allow_lds(int64_t max_bytes) {
+    static const bool done = [max_bytes] {
+        if (max_bytes > 64 * 1024)
+            check(hipFuncSetAttribute(reinterpret_cast<const void*>(Kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      int(max_bytes)),
+                  "dynamic LDS attribute");
+        return true;
+    }();
+    (void)done;
 }
 }  // namespace
 
@@ -1696,20 +1788,20 @@ void launch_qr(const QrDev* tasks, int64_t num, int64_t lds_bytes, int variant, 
     if (num <= 0) return;
     if (variant == 1) {
         if (dtype == 0) {
-            allow_lds(qr_kernel_blocked<double>, QB_LDS);
+            allow_lds<qr_kernel_blocked<double>>(QB_LDS);
             hipLaunchKernelGGL(qr_kernel_blocked<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
                                tasks);
         } else {
-            allow_lds(qr_kernel_blocked<float>, QB_LDS);
+            allow_lds<qr_kernel_blocked<float>>(QB_LDS);
             hipLaunchKernelGGL(qr_kernel_blocked<float>, dim3(unsigned(num)), dim3(QR_WG), size_t(QB_LDS), stream,
                                tasks);
         }
     } else if (lds_bytes > 0) {
         if (dtype == 0) {
-            allow_lds(qr_kernel_lds<double>, lds_bytes);
+            allow_lds<qr_kernel_lds<double>>(QR_LDS_MAX);
             hipLaunchKernelGGL(qr_kernel_lds<double>, dim3(unsigned(num)), dim3(QR_WG), size_t(lds_bytes), stream, tasks);
         } else {
-            allow_lds(qr_kernel_lds<float>, lds_bytes);
+            allow_lds<qr_kernel_lds<float>>(QR_LDS_MAX);
             hipLaunchKernelGGL(qr_kernel_lds<float>, dim3(unsigned(num)), dim3(QR_WG), size_t(lds_bytes), stream, tasks);
         }
     } else {
@@ -1727,7 +1819,7 @@ void launch_svd_blk(const SvdDev* tasks, int64_t num, hipStream_t stream) {
     // block exchange ((waves + 1) x 64 x RPT), at least the 32 KiB of the
     // recovery tiles
     const size_t lds = std::max<size_t>(size_t((RPT / 2 + 1) * 64 * RPT), 2 * 32 * 64) * sizeof(double);
-    allow_lds(svd_kernel_blk<T, RPT>, int64_t(lds));
+    allow_lds<svd_kernel_blk<T, RPT>>(int64_t(lds));
     hipLaunchKernelGGL((svd_kernel_blk<T, RPT>), dim3(unsigned(num)), dim3(32 * RPT), lds, stream, tasks);
 }
 
@@ -1748,11 +1840,11 @@ void launch_svd(const SvdDev* tasks, int64_t num, int64_t lds_bytes, int rpt, in
         }
     } else if (lds_bytes > 0) {
         if (dtype == 0) {
-            allow_lds(svd_kernel_lds<double>, lds_bytes);
+            allow_lds<svd_kernel_lds<double>>(SVD_LDS_MAX);
             hipLaunchKernelGGL(svd_kernel_lds<double>, dim3(unsigned(num)), dim3(SVD_WG), size_t(lds_bytes), stream,
                                tasks);
         } else {
-            allow_lds(svd_kernel_lds<float>, lds_bytes);
+            allow_lds<svd_kernel_lds<float>>(SVD_LDS_MAX);
             hipLaunchKernelGGL(svd_kernel_lds<float>, dim3(unsigned(num)), dim3(SVD_WG), size_t(lds_bytes), stream,
                                tasks);
         }
