@@ -809,7 +809,12 @@ __device__ inline void qb_apply(const double* V, int rows, int nb, const double*
             const gptr<const double> xc = X + int64_t(cb + (c < ncol ? c : 0)) * mx + p0;
             const double* vj = V + int64_t(j < nb ? j : 0) * rows;
             const int rl = lane / 16;
+            // four independent accumulator chains keep the MFMA pipeline fed
+            // (one chain would wait out each MFMA's latency)
             d4 acc = {0.0, 0.0, 0.0, 0.0};
+            d4 acc4[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc4[q] = acc;
             for (int r0 = 0; r0 < rows_full; r0 += 4 * RS) {
                 double av[RS], bv[RS];
 #pragma unroll
@@ -817,8 +822,10 @@ __device__ inline void qb_apply(const double* V, int rows, int nb, const double*
 #pragma unroll
                 for (int u = 0; u < RS; u++) av[u] = vj[r0 + 4 * u + rl];
 #pragma unroll
-                for (int u = 0; u < RS; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc, 0, 0, 0);
+                for (int u = 0; u < RS; u++)
+                    acc4[u % 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc4[u % 4], 0, 0, 0);
             }
+            acc = (acc4[0] + acc4[1]) + (acc4[2] + acc4[3]);
             for (int r0 = rows_full; r0 < rows; r0 += 16) {
                 double av[4], bv[4];
 #pragma unroll
@@ -1113,6 +1120,17 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
     bool converged = kx <= 1;
     for (int sweep = 0; sweep < tk.max_sweeps && kx > 1; sweep++) {
         if (tid == 0) *flag = 0;
+        // exact squared column norms in sig[], carried through the sweep's
+        // rotations (one dot product and one reduction per pair instead of
+        // three); a sweep without rotations tested every pair on these
+        // exact values
+        for (int c = g; c < kx; c += NG) {
+            const double* xc = X + int64_t(c) * mx;
+            double sq = 0;
+            for (int r = gl; r < mx; r += G) sq += xc[r] * xc[r];
+            sq = group_sum(sq, G);
+            if (gl == 0) sig[c] = sq;
+        }
         __syncthreads();
         for (int round = 0; round < P - 1; round++) {
             for (int pi = g; pi < P / 2; pi += NG) {
@@ -1129,7 +1147,8 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                 // the pair's columns stay in registers between the dot
                 // products and the rotation (one read, one write per element)
                 double ra[RPT], rb[RPT];
-                double saa = 0, sbb = 0, sab = 0;
+                const double saa = sig[a], sbb = sig[b];
+                double sab = 0;
                 if (cached) {
 #pragma unroll
                     for (int i = 0; i < RPT; i++) {
@@ -1137,24 +1156,15 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                         const double u = r < mx ? xa[r] : 0.0, w = r < mx ? xb[r] : 0.0;
                         ra[i] = u;
                         rb[i] = w;
-                        saa += u * u;
-                        sbb += w * w;
                         sab += u * w;
                     }
                 } else {
-                    for (int r = gl; r < mx; r += G) {
-                        const double u = xa[r], w = xb[r];
-                        saa += u * u;
-                        sbb += w * w;
-                        sab += u * w;
-                    }
+                    for (int r = gl; r < mx; r += G) sab += xa[r] * xb[r];
                 }
-                saa = group_sum(saa, G);
-                sbb = group_sum(sbb, G);
                 sab = group_sum(sab, G);
                 if (saa > live && sbb > live && sab != 0.0 && fabs(sab) > tol * sqrt(saa) * sqrt(sbb)) {
-                    double cs, sn;
-                    jacobi_rotation(saa, sbb, sab, cs, sn);
+                    double cs, sn, t;
+                    jacobi_rotation(saa, sbb, sab, cs, sn, t);
                     if (cached) {
 #pragma unroll
                         for (int i = 0; i < RPT; i++) {
@@ -1178,7 +1188,11 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
                         va[r] = cs * u - sn * w;
                         vb[r] = sn * u + cs * w;
                     }
-                    if (gl == 0) *flag = 1;
+                    if (gl == 0) {
+                        sig[a] = jacobi_tracked(saa, saa - t * sab);
+                        sig[b] = jacobi_tracked(sbb, sbb + t * sab);
+                        *flag = 1;
+                    }
                 }
             }
             __syncthreads();
@@ -1546,19 +1560,28 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
         // previous one, wave 0's bottom becomes wave 1's top and the last
         // wave's top its own bottom), each followed by the 8 cross rounds
         for (int b = 1; b < 2 * NWV - 1; b++) {
+            // the lane's LDS column base, made opaque each round: the row
+            // offsets i * WAVE then fold into the LDS instructions' immediate
+            // offsets instead of being hoisted out of the loops as RPT live
+            // addresses (which spilled to scratch at RPT = 24)
+            int lb = lane;
+            asm volatile("" : "+v"(lb));
+            double* const own = xbuf + w * RPT * WAVE + lb;
+            double* const spare = xbuf + NWV * RPT * WAVE + lb;
             // bottoms out (wave 0's also into the spare slot NWV)
 #pragma unroll
-            for (int i = 0; i < RPT; i++) xbuf[(w * RPT + i) * WAVE + lane] = xb[i];
+            for (int i = 0; i < RPT; i++) own[i * WAVE] = xb[i];
             if (gl == 0) ibuf[w][pl] = cb;
             if (w == 0) {
 #pragma unroll
-                for (int i = 0; i < RPT; i++) xbuf[(NWV * RPT + i) * WAVE + lane] = xb[i];
+                for (int i = 0; i < RPT; i++) spare[i * WAVE] = xb[i];
                 if (gl == 0) ibuf[NWV][pl] = cb;
             }
             __syncthreads();
             if (w + 1 < NWV) {
+                const double* next = xbuf + (w + 1) * RPT * WAVE + lb;
 #pragma unroll
-                for (int i = 0; i < RPT; i++) xb[i] = xbuf[((w + 1) * RPT + i) * WAVE + lane];
+                for (int i = 0; i < RPT; i++) xb[i] = next[i * WAVE];
                 cb = ibuf[w + 1][pl];
             } else {
 #pragma unroll
@@ -1568,16 +1591,17 @@ __global__ __launch_bounds__(32 * RPT) void svd_kernel_blk(const SvdDev* __restr
             __syncthreads();
             // tops out
 #pragma unroll
-            for (int i = 0; i < RPT; i++) xbuf[(w * RPT + i) * WAVE + lane] = xa[i];
+            for (int i = 0; i < RPT; i++) own[i * WAVE] = xa[i];
             if (gl == 0) ibuf[w][pl] = ca;
             __syncthreads();
             if (w >= 2) {
+                const double* prev = xbuf + (w - 1) * RPT * WAVE + lb;
 #pragma unroll
-                for (int i = 0; i < RPT; i++) xa[i] = xbuf[((w - 1) * RPT + i) * WAVE + lane];
+                for (int i = 0; i < RPT; i++) xa[i] = prev[i * WAVE];
                 ca = ibuf[w - 1][pl];
             } else if (w == 1) {
 #pragma unroll
-                for (int i = 0; i < RPT; i++) xa[i] = xbuf[(NWV * RPT + i) * WAVE + lane];
+                for (int i = 0; i < RPT; i++) xa[i] = spare[i * WAVE];
                 ca = ibuf[NWV][pl];
             }
             __syncthreads();
