@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--chunk", type=int, default=256, help="configurations per lockstep pass of one lane")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=8, help="concurrent lockstep lanes (host thread + HIP stream each)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's); with fewer hardware "
+                         "queues than lanes the lanes' streams share queues and a long kernel of one lane holds "
+                         "back the launches queued behind it")
     ap.add_argument("--sampler-lanes", type=int, default=4,
                     help="lanes of the sampling legs (sweep, ESS, direct, gradient descent): their batched ops "
                          "split the chains, so fewer, larger lanes do better there")
@@ -425,6 +429,11 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            sys.exit("--hw-queues must be in [1, 32]")
+        # read by the HIP runtime when it initialises, which nothing has done yet
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import numpy as np
     import torch
 
@@ -592,7 +601,8 @@ def main():
                                f"{list(spec.segments)} around tV charge backgrounds, Dc={spec.cut}, N={spec.N}, "
                                f"{B} configurations per step drawn uniformly at fixed particle number, "
                                f"batched lockstep engine, contraction GEMMs {args.precision}, QR/SVD fp64",
-                   "batch": B, "threads": threads, "lanes": args.lanes},
+                   "batch": B, "threads": threads, "lanes": args.lanes,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity_gate": gate,

@@ -597,6 +597,55 @@ __device__ inline double jacobi_tracked(double old_n, double new_n) {
 // R is written out first, then Q is formed in place from the reflectors
 // (backward accumulation, as LAPACK's orgqr does).  X lives in LDS when it
 // fits, otherwise in global scratch.
+// Householder reflector of a column with diagonal entry alpha and squared
+// norm sigma below it: H = I - tau v v^T, v = (1, x / (alpha - beta)),
+// H (alpha, x) = (beta, 0).  Every thread computes the same values.
+__device__ inline void householder(double alpha, double sigma, double& tau, double& beta) {
+    if (sigma == 0.0) {
+        tau = 0.0;
+        beta = alpha;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        tau = (beta - alpha) / beta;
+    }
+}
+
+// Apply the reflector of column j (unit diagonal implied, entries below j in
+// cj) to columns j + 1 .. nc - 1 of X (column stride ld), one group of G
+// lanes per column.  The group that updates column j + 1 also sums the
+// squares of its new entries below row j + 1 (the next reflector's sigma)
+// into *next (written when j + 1 < nc).
+__device__ inline void qr_update_next(const double* cj, double* X, int ld, int j, int nc, double tj, int g, int gl,
+                                      int G, int NG, double* next) {
+    const int rows = ld;
+    if (tj != 0.0) {
+        for (int c = j + 1 + g; c < nc; c += NG) {
+            double* cc = X + int64_t(c) * ld;
+            double d = 0;
+            for (int r = j + 1 + gl; r < rows; r += G) d += cj[r] * cc[r];
+            d = group_sum(d, G) + cc[j];
+            const double f = tj * d;
+            if (gl == 0) cc[j] -= f;
+            double sq = 0;
+            for (int r = j + 1 + gl; r < rows; r += G) {
+                const double v = cc[r] - f * cj[r];
+                cc[r] = v;
+                if (r > j + 1) sq += v * v;
+            }
+            if (c == j + 1) {
+                sq = group_sum(sq, G);
+                if (gl == 0) *next = sq;
+            }
+        }
+    } else if (g == 0 && j + 1 < nc) {
+        const double* cc = X + int64_t(j + 1) * ld;
+        double sq = 0;
+        for (int r = j + 2 + gl; r < rows; r += G) sq += cc[r] * cc[r];
+        sq = group_sum(sq, G);
+        if (gl == 0) *next = sq;
+    }
+}
+
 template<typename T>
 __device__ __forceinline__ void qr_body(const QrDev& tk, double* X, double* tau, double* red, double* bc) {
     const int m = tk.m, n = tk.n, lq = tk.lq;
@@ -609,45 +658,31 @@ __device__ __forceinline__ void qr_body(const QrDev& tk, double* X, double* tau,
         X[e] = double(lq ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
     }
     __syncthreads();
+    // squared norm below the diagonal of the column to reduce: a block sum
+    // for column 0, then from the group that updates the next column
+    double sigma;
+    {
+        double s = 0;
+        for (int r = 1 + tid; r < mx; r += QR_WG) s += X[r] * X[r];
+        sigma = block_sum<QR_WG>(s, red);
+    }
     for (int j = 0; j < k; j++) {
         double* cj = X + int64_t(j) * mx;
-        double s = 0;
-        for (int r = j + 1 + tid; r < mx; r += QR_WG) s += cj[r] * cj[r];
-        const double sigma = block_sum<QR_WG>(s, red);
-        if (tid == 0) {
-            const double alpha = cj[j];
-            if (sigma == 0.0) {
-                bc[0] = 0.0;
-                bc[1] = alpha;
-            } else {
-                const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-                bc[0] = (beta - alpha) / beta;
-                bc[1] = beta;
-            }
-        }
-        __syncthreads();
-        const double tj = bc[0], beta = bc[1];
+        double tj, beta;
+        householder(cj[j], sigma, tj, beta);
         if (tj != 0.0) {
             const double scale = 1.0 / (cj[j] - beta);
             for (int r = j + 1 + tid; r < mx; r += QR_WG) cj[r] *= scale;
         }
+        // every thread has read cj[j]; the reflector below it is scaled
         __syncthreads();
         if (tid == 0) {
             cj[j] = beta;
             tau[j] = tj;
         }
-        if (tj != 0.0) {
-            for (int c = j + 1 + g; c < nx; c += NG) {
-                double* cc = X + int64_t(c) * mx;
-                double d = 0;
-                for (int r = j + 1 + gl; r < mx; r += G) d += cj[r] * cc[r];
-                d = group_sum(d, G) + cc[j];
-                const double f = tj * d;
-                if (gl == 0) cc[j] -= f;
-                for (int r = j + 1 + gl; r < mx; r += G) cc[r] -= f * cj[r];
-            }
-        }
+        qr_update_next(cj, X, mx, j, nx, tj, g, gl, G, NG, bc);
         __syncthreads();
+        sigma = bc[0];
     }
     T* f1 = static_cast<T*>(tk.f1);
     T* f2 = static_cast<T*>(tk.f2);
@@ -707,7 +742,7 @@ template<typename T>
 __global__ __launch_bounds__(QR_WG) void qr_kernel_lds(const QrDev* __restrict__ tasks) {
     extern __shared__ double smem[];
     __shared__ double red[QR_WG / WAVE];
-    __shared__ double bc[2];
+    __shared__ double bc[1];
     const QrDev tk = tasks[blockIdx.x];
     const int mx = tk.lq ? tk.n : tk.m, nx = tk.lq ? tk.m : tk.n;
     qr_body<T>(tk, smem, smem + int64_t(mx) * nx, red, bc);
@@ -716,7 +751,7 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_lds(const QrDev* __restrict__
 template<typename T>
 __global__ __launch_bounds__(QR_WG) void qr_kernel_glb(const QrDev* __restrict__ tasks) {
     __shared__ double red[QR_WG / WAVE];
-    __shared__ double bc[2];
+    __shared__ double bc[1];
     const QrDev tk = tasks[blockIdx.x];
     const int mx = tk.lq ? tk.n : tk.m, nx = tk.lq ? tk.m : tk.n;
     qr_body<T>(tk, tk.work, tk.work + int64_t(mx) * nx, red, bc);
@@ -905,7 +940,7 @@ template<typename T>
 __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restrict__ tasks) {
     extern __shared__ double sm[];
     __shared__ double red[QR_WG / WAVE];
-    __shared__ double bc[2];
+    __shared__ double bc[1];
     const QrDev tk = tasks[blockIdx.x];
     const int m = tk.m, n = tk.n, lq = tk.lq;
     const int mx = lq ? n : m, nx = lq ? m : n, k = mx < nx ? mx : nx;
@@ -933,24 +968,16 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
             V[e] = X[int64_t(p0 + c) * mx + p0 + r];
         }
         __syncthreads();
+        double sigma;
+        {
+            double sq = 0;
+            for (int r = 1 + tid; r < rows; r += QR_WG) sq += V[r] * V[r];
+            sigma = block_sum<QR_WG>(sq, red);
+        }
         for (int jj = 0; jj < nb; jj++) {
             double* cj = V + int64_t(jj) * rows;
-            double sq = 0;
-            for (int r = jj + 1 + tid; r < rows; r += QR_WG) sq += cj[r] * cj[r];
-            const double sigma = block_sum<QR_WG>(sq, red);
-            if (tid == 0) {
-                const double alpha = cj[jj];
-                if (sigma == 0.0) {
-                    bc[0] = 0.0;
-                    bc[1] = alpha;
-                } else {
-                    const double beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-                    bc[0] = (beta - alpha) / beta;
-                    bc[1] = beta;
-                }
-            }
-            __syncthreads();
-            const double tj = bc[0], beta = bc[1];
+            double tj, beta;
+            householder(cj[jj], sigma, tj, beta);
             if (tj != 0.0) {
                 const double scale = 1.0 / (cj[jj] - beta);
                 for (int r = jj + 1 + tid; r < rows; r += QR_WG) cj[r] *= scale;
@@ -961,18 +988,9 @@ __global__ __launch_bounds__(QR_WG) void qr_kernel_blocked(const QrDev* __restri
                 taus[jj] = tj;
                 tau[p0 + jj] = tj;
             }
-            if (tj != 0.0) {
-                for (int c = jj + 1 + g; c < nb; c += NG) {
-                    double* cc = V + int64_t(c) * rows;
-                    double d = 0;
-                    for (int r = jj + 1 + gl; r < rows; r += G) d += cj[r] * cc[r];
-                    d = group_sum(d, G) + cc[jj];
-                    const double f = tj * d;
-                    if (gl == 0) cc[jj] -= f;
-                    for (int r = jj + 1 + gl; r < rows; r += G) cc[r] -= f * cj[r];
-                }
-            }
+            qr_update_next(cj, V, rows, jj, nb, tj, g, gl, G, NG, bc);
             __syncthreads();
+            sigma = bc[0];
         }
         // panel back to X (R on and above the diagonal, reflectors below),
         // then V as unit lower for the block reflector
