@@ -37,9 +37,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--chunk", type=int, default=256, help="configurations per lockstep pass of one lane")
+    ap.add_argument("--chunk", type=int, default=128, help="configurations per lockstep pass of one lane")
+    ap.add_argument("--timing-chunk", type=int, default=256,
+                    help="configurations per launch of the one-lane kernel timing pass after the timed steps")
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--lanes", type=int, default=8, help="concurrent lockstep lanes (host thread + HIP stream each)")
+    ap.add_argument("--lanes", type=int, default=16, help="concurrent lockstep lanes (host thread + HIP stream each)")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's); with fewer hardware "
                          "queues than lanes the lanes' streams share queues and a long kernel of one lane holds "
@@ -212,7 +214,7 @@ def measured_traffic(kernel, args):
     WRITE_SIZE profile of the same workload and launch size (rocprofv3 --pmc
     runs scaled by the counters' reading for a device copy of known size)."""
     path = os.path.join(ROOT, "profiles", "c4_traffic.json")
-    if args.workload != "C4" or args.chunk != 256 or args.precision != "fp64" or not os.path.exists(path):
+    if args.workload != "C4" or args.timing_chunk != 256 or args.precision != "fp64" or not os.path.exists(path):
         return None
     with open(path) as f:
         k = json.load(f)["kernels"].get(kernel)
@@ -496,21 +498,24 @@ def main():
     psi = np.concatenate(psis)
     st = eng.stats()
     value = B * args.steps / elapsed
-    # kernel durations: one chunk on one lane, outside the timed region
+    # kernel durations: one chunk of --timing-chunk configurations on one
+    # lane, outside the timed region
     i0 = (args.warmup + args.steps - 1) * B
+    n_timing = min(B, args.timing_chunk)
     eng.set_lanes(1)
+    eng.set_chunk(n_timing)
     eng.reset_stats()
     eng.set_profile(True)
-    eng.evaluate(configs[i0:i0 + min(B, args.chunk)])
+    eng.evaluate(configs[i0:i0 + n_timing])
     torch.cuda.synchronize()
     eng.set_profile(False)
     ks = eng.kernel_stats()
+    eng.set_chunk(args.chunk)
     eng.set_lanes(args.lanes)
     # the legs below run their own samplers and engines: give this engine's
     # working memory back first
     eng.trim()
-    print(f"[bench] kernel timing pass: {min(B, args.chunk)} configurations on one lane", file=sys.stderr,
-          flush=True)
+    print(f"[bench] kernel timing pass: {n_timing} configurations on one lane", file=sys.stderr, flush=True)
     dom = dominant_kernel(ks)
     roof = dom[1] if dom else None
     if roof:
@@ -618,7 +623,9 @@ def main():
                         if (ess or ess_leg)["sweep"]["reference_chain"] else
                         "; reference_chain false: doubly occupied bonds keep psi instead of tetragono's "
                         "hole(()) refresh (a cheaper, different chain)")},
-        "extra": {"kernel_timing": f"one lane, {min(B, args.chunk)} configurations, HIP events on the launch stream",
+        "extra": {"kernel_timing": (f"one lane, {n_timing} configurations per launch (the timed steps launch "
+                                    f"{min(args.chunk, B)} per lane on {args.lanes} lanes), HIP events on the "
+                                    "launch stream"),
                   "kernel_ms": {k: v["ms"] for k, v in ks.items()},
                   "kernel_rates": kernel_rates(ks),
                   "kernel_launches": {k: v["launches"] for k, v in ks.items()},

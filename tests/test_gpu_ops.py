@@ -318,3 +318,35 @@ def test_svd_structurally_rank_deficient(scale, rows, cols, rank):
     assert np.abs(got[:rank] - ref[:rank]).max() <= 1e-13 * ref[0]
     assert np.all(got[rank:] <= 1e-13 * ref[0])
     assert np.abs(um @ sm @ vm - x).max() <= 1e-12 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("rows,cols", [(192, 36), (36, 192), (120, 40), (100, 8), (64, 2), (90, 44)])
+@pytest.mark.parametrize("graded", [False, True])
+def test_svd_tall_lds_tier(rows, cols, graded):
+    """Tall and wide blocks of the LDS tier (working set at most 76 KiB, the
+    shapes that carry most of C4's LDS-tier Jacobi work), where the tracked
+    column norms see the longest columns: singular values equal LAPACK's to
+    1e-13 of the largest, U and V are orthonormal (V is accumulated here, not
+    recovered) and U S V rebuilds the block, for random and graded
+    (1 .. 1e-12) spectra."""
+    from tnsp_amd import TAT
+
+    rng = np.random.default_rng(rows * 1000 + cols)
+    x = rng.standard_normal((rows, cols))
+    if graded:
+        u0, s0, v0 = np.linalg.svd(x, full_matrices=False)
+        x = (u0 * (s0 * np.logspace(0, -12, s0.size))) @ v0
+    a = TAT.No.D.Tensor(["a", "b"], [rows, cols])
+    a.storage = x.reshape(-1)
+    u, s, v = a.svd({"a"}, "U", "V", "SU", "SV")
+    k = s.edges[0].dimension()
+    assert k == min(rows, cols)
+    um = np.asarray(storage(u.transpose(["a", "U"]))).reshape(rows, k)
+    sm = np.asarray(storage(s.transpose(["SU", "SV"]))).reshape(k, k)
+    vm = np.asarray(storage(v.transpose(["V", "b"]))).reshape(k, cols)
+    ref = np.linalg.svd(x, compute_uv=False)
+    got = np.sort(np.abs(np.diag(sm)))[::-1]
+    assert np.abs(got - ref).max() <= 1e-13 * ref[0]
+    assert np.abs(um.T @ um - np.eye(k)).max() <= 1e-12
+    assert np.abs(vm @ vm.T - np.eye(k)).max() <= 1e-12
+    assert np.abs(um @ sm @ vm - x).max() <= 1e-12 * np.abs(x).max()
