@@ -152,7 +152,8 @@ int tns_engine_create(int kind, int L1, int L2, const tns_shape* const* sites, c
 int tns_engine_set_chunk(tns_engine* e, int64_t chunk);
 /* lanes: independent lockstep pipelines over the same PEPS, each on its own
  * host thread and HIP stream; configurations are split over them so one
- * lane's host planning and decomposition tails overlap the others' kernels */
+ * lane's host planning and decomposition tails overlap the others' kernels
+ * (1 <= lanes <= 32) */
 int tns_engine_set_lanes(tns_engine* e, int lanes);
 /* precision of the contraction GEMMs of a float64 engine: 0 fp64, 1 fp32,
  * 2 split-bf16 x3, 3 split-bf16 x6 (fp32 accumulation); operands are

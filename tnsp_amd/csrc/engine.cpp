@@ -808,7 +808,7 @@ int tns_engine_autotune(tns_engine* e, int64_t nconf, const int32_t* occupations
 
 int tns_engine_set_lanes(tns_engine* e, int lanes) {
     TNS_TRY({
-        if (lanes < 1 || lanes > 16) throw tns::Error("lanes must be in [1, 16]");
+        if (lanes < 1 || lanes > 32) throw tns::Error("lanes must be in [1, 32]");
         if (lanes < int(e->lanes.size())) {
             e->lanes.resize(size_t(lanes));
             for (size_t i = size_t(lanes); i < e->streams.size(); i++)
