@@ -156,18 +156,39 @@ void Shape::finalize() {
 // ------------------------------------------------------------------------
 // edge_operator (edge_operator.hpp:35-692)
 // ------------------------------------------------------------------------
+namespace {
+struct EdgeOpScratch {
+    std::vector<EdgeView> edges_bs, edges_as, edges_bt, edges_bm;
+    std::vector<int32_t> names_as, names_bm;
+    std::vector<int> split_flags, merge_flags, d2s, group_start, active, odd_list;
+    std::vector<GroupTable> split_tab, merge_tab;
+    std::vector<bool> is_split, am_is_merge, group_merged;
+    std::vector<char> rev_before, rev_after, mk_split, mk_rb, mk_ra, mk_merge, rev_flag;
+    std::vector<int64_t> pos_bm, pos_as, dims_bm, dims_as, pos_am, off_am, pos_bs, off_bs, pos_src;
+    std::vector<int64_t> lead_am, lead_bm, lead_bs, lead_as, tdims, tss, tds, aidx;
+};
+EdgeOpScratch& edge_op_scratch() {
+    thread_local EdgeOpScratch s;
+    return s;
+}
+}  // namespace
+
 void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, std::vector<PermTask>& tasks,
                         int32_t src_buf, int32_t dst_buf) {
     const Kind& k = src.kind();
     const bool fermi = k.is_fermi;
     const int rank_bs = src.rank();
+    // per-thread working vectors, reused from call to call (the planner runs
+    // thousands of these per configuration; their allocations dominated)
+    EdgeOpScratch& S = edge_op_scratch();
 
     // edges are viewed (segments + arrow), not copied, until the result
     // shape is built
     // step 1: cut
     std::vector<Edge> cut_store;
     cut_store.reserve(args.cut.size());
-    std::vector<EdgeView> edges_bs(rank_bs);
+    auto& edges_bs = S.edges_bs;
+    edges_bs.assign(rank_bs, EdgeView());
     for (int i = 0; i < rank_bs; i++) edges_bs[i] = EdgeView(src.edges[i]);
     bool has_cut = false;
     for (auto& c : args.cut) {
@@ -193,11 +214,16 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     }
 
     // step 2: split
-    std::vector<int32_t> names_as;
-    std::vector<int> split_flags;
-    std::vector<EdgeView> edges_as;
-    std::vector<GroupTable> split_tab(rank_bs);
-    std::vector<bool> is_split(rank_bs, false);
+    auto& names_as = S.names_as;
+    auto& split_flags = S.split_flags;
+    auto& edges_as = S.edges_as;
+    auto& split_tab = S.split_tab;  // an entry is rebuilt before it is read
+    auto& is_split = S.is_split;
+    names_as.clear();
+    split_flags.clear();
+    edges_as.clear();
+    if (split_tab.size() < size_t(rank_bs)) split_tab.resize(size_t(rank_bs));
+    is_split.assign(rank_bs, false);
     for (int i = 0; i < rank_bs; i++) {
         const SplitSpec* sp = nullptr;
         for (auto& s : args.split)
@@ -256,8 +282,10 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     const int rank_t = int(names_as.size());
 
     // step 3: reverse before transpose
-    std::vector<char> rev_before(rank_t, 0);
-    std::vector<EdgeView> edges_bt = edges_as;
+    auto& rev_before = S.rev_before;
+    rev_before.assign(rank_t, 0);
+    auto& edges_bt = S.edges_bt;
+    edges_bt.assign(edges_as.begin(), edges_as.end());
     if (fermi) {
         for (int i = 0; i < rank_t; i++) {
             if (contains(args.reversed, names_as[i])) {
@@ -270,9 +298,12 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     // step 4: names down to up
     const auto& names_am = args.new_names;
     const int rank_am = int(names_am.size());
-    std::vector<int32_t> names_bm;
-    std::vector<int> merge_flags;
-    std::vector<bool> am_is_merge(rank_am, false);
+    auto& names_bm = S.names_bm;
+    auto& merge_flags = S.merge_flags;
+    auto& am_is_merge = S.am_is_merge;
+    names_bm.clear();
+    merge_flags.clear();
+    am_is_merge.assign(rank_am, false);
     for (int j = 0; j < rank_am; j++) {
         const MergeSpec* ms = nullptr;
         for (auto& m : args.merge)
@@ -289,7 +320,8 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
         }
     }
     if (int(names_bm.size()) != rank_t) throw Error("Tensor to transpose with Different Rank");
-    std::vector<int> d2s(rank_t);
+    auto& d2s = S.d2s;
+    d2s.assign(rank_t, 0);
     for (int i = 0; i < rank_t; i++) {
         int f = -1;
         for (int s = 0; s < rank_t; s++)
@@ -299,17 +331,22 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     }
 
     // step 5: reverse before merge, merge tables, result edges
-    std::vector<EdgeView> edges_bm(rank_t);
+    auto& edges_bm = S.edges_bm;
+    edges_bm.assign(rank_t, EdgeView());
     for (int i = 0; i < rank_t; i++) edges_bm[i] = edges_bt[d2s[i]];
-    std::vector<char> rev_after(rank_t, 0);
+    auto& rev_after = S.rev_after;
+    rev_after.assign(rank_t, 0);
     const bool any_merge = !args.merge.empty();
     out = Shape();
     out.kind_id = src.kind_id;
     out.names = names_am;
     out.edges.resize(rank_am);
-    std::vector<GroupTable> merge_tab(rank_am);
-    std::vector<bool> group_merged(rank_am, false);
-    std::vector<int> group_start(rank_am + 1, 0);
+    auto& merge_tab = S.merge_tab;  // an entry is rebuilt before it is read
+    if (merge_tab.size() < size_t(rank_am)) merge_tab.resize(size_t(rank_am));
+    auto& group_merged = S.group_merged;
+    group_merged.assign(rank_am, false);
+    auto& group_start = S.group_start;
+    group_start.assign(rank_am + 1, 0);
     {
         int start = 0;
         for (int j = 0; j < rank_am; j++) {
@@ -378,17 +415,23 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
     out.finalize();
 
     // marks: true => apply that half sign
-    std::vector<char> mk_split(rank_bs), mk_rb(rank_t), mk_ra(rank_t), mk_merge(rank_am);
+    auto &mk_split = S.mk_split, &mk_rb = S.mk_rb, &mk_ra = S.mk_ra, &mk_merge = S.mk_merge;
+    mk_split.assign(rank_bs, 0);
+    mk_rb.assign(rank_t, 0);
+    mk_ra.assign(rank_t, 0);
+    mk_merge.assign(rank_am, 0);
     for (int i = 0; i < rank_bs; i++) mk_split[i] = contains(args.ex_split, src.names[i]) != args.apply_parity;
     for (int i = 0; i < rank_t; i++) mk_rb[i] = contains(args.ex_rev_before, names_as[i]) != args.apply_parity;
     for (int i = 0; i < rank_t; i++) mk_ra[i] = contains(args.ex_rev_after, names_bm[i]) != args.apply_parity;
     for (int j = 0; j < rank_am; j++) mk_merge[j] = contains(args.ex_merge, names_am[j]) != args.apply_parity;
-    std::vector<char> rev_flag(rank_t);
+    auto& rev_flag = S.rev_flag;
+    rev_flag.assign(rank_t, 0);
     for (int i = 0; i < rank_t; i++)
         rev_flag[i] = char((rev_after[i] && mk_ra[i]) ^ (rev_before[d2s[i]] && mk_rb[d2s[i]]));
 
     // main loop over before-merge segment tuples with zero total charge
-    std::vector<int> active;
+    auto& active = S.active;
+    active.clear();
     Sym const_sum;
     bool const_par = false;
     for (int i = 0; i < rank_t; i++) {
@@ -400,14 +443,19 @@ void plan_edge_operator(const Shape& src, const EdgeOpArgs& args, Shape& out, st
             active.push_back(i);
         }
     }
-    std::vector<int64_t> pos_bm(rank_t, 0), pos_as(rank_t), dims_bm(rank_t), dims_as(rank_t);
-    std::vector<int64_t> pos_am(rank_am), off_am(rank_am), pos_bs(rank_bs), off_bs(rank_bs), pos_src(rank_bs);
-    std::vector<int64_t> lead_am(rank_am), lead_bm(rank_t), lead_bs(rank_bs), lead_as(rank_t);
-    std::vector<int64_t> tdims(rank_t), tss(rank_t), tds(rank_t);
-    std::vector<int> odd_list;
+    auto &pos_bm = S.pos_bm, &pos_as = S.pos_as, &dims_bm = S.dims_bm, &dims_as = S.dims_as;
+    auto &pos_am = S.pos_am, &off_am = S.off_am, &pos_bs = S.pos_bs, &off_bs = S.off_bs, &pos_src = S.pos_src;
+    auto &lead_am = S.lead_am, &lead_bm = S.lead_bm, &lead_bs = S.lead_bs, &lead_as = S.lead_as;
+    auto &tdims = S.tdims, &tss = S.tss, &tds = S.tds;
+    for (auto* v : {&pos_bm, &pos_as, &dims_bm, &dims_as, &lead_bm, &lead_as, &tdims, &tss, &tds}) v->assign(rank_t, 0);
+    for (auto* v : {&pos_am, &off_am, &lead_am}) v->assign(rank_am, 0);
+    for (auto* v : {&pos_bs, &off_bs, &pos_src, &lead_bs}) v->assign(rank_bs, 0);
+    auto& odd_list = S.odd_list;
+    odd_list.clear();
     odd_list.reserve(rank_t);
     const int na = int(active.size());
-    std::vector<int64_t> aidx(na, 0);
+    auto& aidx = S.aidx;
+    aidx.assign(na, 0);
     // the odometer runs over all active edges but the last; the last edge's
     // segment is the one whose charge closes the total (charges are unique
     // within an edge), so forbidden tuples are never visited
