@@ -17,6 +17,15 @@ def test_bench_refuses_several_gpus():
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
 
 
+def test_bench_refuses_hw_queues_beyond_32():
+    """--hw-queues sets GPU_MAX_HW_QUEUES before HIP initialises; values
+    outside [1, 32] end the run with a message before any device is touched."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--hw-queues", "40"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode != 0
+    assert "--hw-queues must be in [1, 32]" in r.stderr
+
+
 def _bench():
     import importlib.util
 
