@@ -646,13 +646,23 @@ __device__ inline void qr_update_next(const double* cj, double* X, int ld, int j
     }
 }
 
+// Lanes per column group of the Householder updates: the widest group (of
+// 16, 32, 64, no wider than the column needs) that still gives every
+// column to be updated a group of its own, so that a reflector's update is
+// one round of groups instead of several in a row.
+__device__ inline int qr_group_lanes(int rows, int cols) {
+    int G = group_lanes(rows);
+    while (G > 16 && QR_WG / G < cols) G /= 2;
+    return G;
+}
+
 template<typename T>
 __device__ __forceinline__ void qr_body(const QrDev& tk, double* X, double* tau, double* red, double* bc) {
     const int m = tk.m, n = tk.n, lq = tk.lq;
     const int mx = lq ? n : m, nx = lq ? m : n, k = mx < nx ? mx : nx;
     const T* in = static_cast<const T*>(tk.in);
     const int tid = threadIdx.x;
-    const int G = group_lanes(mx), NG = QR_WG / G, g = tid / G, gl = tid % G;
+    const int G = qr_group_lanes(mx, nx), NG = QR_WG / G, g = tid / G, gl = tid % G;
     for (int64_t e = tid; e < int64_t(mx) * nx; e += QR_WG) {
         const int r = int(e % mx), c = int(e / mx);
         X[e] = double(lq ? in[int64_t(c) * n + r] : in[int64_t(r) * n + c]);
