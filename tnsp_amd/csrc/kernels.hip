@@ -1100,13 +1100,19 @@ __device__ __forceinline__ void svd_write(const SvdDev& tk, const double* X, con
 // by one group of G lanes.  X, V (kx x kx) and sigma live in LDS when they
 // fit, otherwise in global scratch.  Singular values sorted descending.
 template<typename T>
-__device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V, double* sig, int* flag) {
+__device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V, double* sig, int* flag, bool lds) {
     const int m = tk.m, n = tk.n;
     const bool trans = m < n;
     const int mx = trans ? n : m, kx = trans ? m : n;
     const T* in = static_cast<const T*>(tk.in);
     const int tid = threadIdx.x;
-    const int G = group_lanes(mx), NG = SVD_WG / G, g = tid / G, gl = tid % G;
+    // lanes per pair group: as for the rows, but in LDS narrower while the
+    // groups would not cover a round's kx / 2 pairs at once (each lane then
+    // keeps more rows of the pair, up to RPT in registers); the long columns
+    // of the global-scratch kernel keep the row-sized groups
+    int G0 = group_lanes(mx);
+    while (lds && G0 > 16 && SVD_WG / G0 < (kx + 1) / 2) G0 /= 2;
+    const int G = G0, NG = SVD_WG / G, g = tid / G, gl = tid % G;
     __shared__ double wmax[SVD_WG / WAVE];
     double amax = 0.0;
     for (int64_t e = tid; e < int64_t(mx) * kx; e += SVD_WG) {
@@ -1143,7 +1149,7 @@ __device__ __forceinline__ void svd_body(const SvdDev& tk, double* X, double* V,
     __syncthreads();
     const int P = kx + (kx & 1);
     const double tol = fmax(1e-15, 2.220446049250313e-16 * mx);
-    constexpr int RPT = 4;
+    constexpr int RPT = 12;
     const bool cached = mx <= RPT * G;
     bool converged = kx <= 1;
     for (int sweep = 0; sweep < tk.max_sweeps && kx > 1; sweep++) {
@@ -1296,7 +1302,7 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_lds(const SvdDev* __restric
     __shared__ int flag;
     const SvdDev tk = tasks[blockIdx.x];
     const int mx = tk.m < tk.n ? tk.n : tk.m, kx = tk.m < tk.n ? tk.m : tk.n;
-    svd_body<T>(tk, smem, smem + int64_t(mx) * kx, smem + int64_t(mx) * kx + int64_t(kx) * kx, &flag);
+    svd_body<T>(tk, smem, smem + int64_t(mx) * kx, smem + int64_t(mx) * kx + int64_t(kx) * kx, &flag, true);
 }
 
 template<typename T>
@@ -1305,7 +1311,7 @@ __global__ __launch_bounds__(SVD_WG) void svd_kernel_glb(const SvdDev* __restric
     const SvdDev tk = tasks[blockIdx.x];
     const int mx = tk.m < tk.n ? tk.n : tk.m, kx = tk.m < tk.n ? tk.m : tk.n;
     double* X = tk.work;
-    svd_body<T>(tk, X, X + int64_t(mx) * kx, X + int64_t(mx) * kx + int64_t(kx) * kx, &flag);
+    svd_body<T>(tk, X, X + int64_t(mx) * kx, X + int64_t(mx) * kx + int64_t(kx) * kx, &flag, false);
 }
 
 
